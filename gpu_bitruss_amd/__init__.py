@@ -1,0 +1,22 @@
+"""MI355X-native bitruss hot path: butterfly counting (bfc), per-edge
+butterfly support (ebfc), h-index bitruss (hidx) and GBiD multi-stage
+peeling (msp), as hand-written HIP kernels for gfx950 behind a C-ABI
+(include/bitruss_hip.h, include/bt_graph.h).
+"""
+from .graph import Graph, WORKLOADS
+from .device import (
+    Device,
+    DeviceGraph,
+    bfc_evpp,
+    bfc_vpp,
+    bitruss_msp,
+    default_device,
+    device_count,
+    g_bitruss_hindex,
+)
+from ._lib import BitrussError, LIB_PATH, COH_PATH
+
+__all__ = [
+    "Graph", "WORKLOADS", "Device", "DeviceGraph", "bfc_vpp", "bfc_evpp", "g_bitruss_hindex",
+    "bitruss_msp", "default_device", "device_count", "BitrussError", "LIB_PATH", "COH_PATH",
+]

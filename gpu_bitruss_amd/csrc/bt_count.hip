@@ -1,0 +1,456 @@
+// Butterfly counting (bfc) and per-edge support (ebfc) for gfx950.
+//
+// Semantics: the reference's vertex-priority counting (CPU cpu_bfc.cpp:24-203,
+// GPU bfc_vpp.cu:33-221): every butterfly is counted once at its vertex of
+// highest priority u (priority = (degree, id)), as the pairs of wedges
+// u-v-w with v, w below u that share the far end w; with cnt[w] such wedges
+// a root contributes C(cnt[w],2) butterflies, and each of its wedges adds
+// cnt[w]-1 to the supports of u-v and v-w.
+//
+// Layout (MI355X design, not the reference's): the CSR is relabelled by
+// priority rank once per call, each row sorted by rank, so for root r the
+// lower-priority neighbours are a row prefix (r_lo[r] entries) and for each
+// such v the valid far ends are the prefix of row v below r (r_p entries).
+// Roots are bucketed by their wedge count W_r:
+//   A  W_r <= 1024 : one 64-lane wavefront per root, packed LDS hash (8 KB)
+//   B  W_r <= 8192 : one 256-thread workgroup per root, LDS hash (64 KB)
+//   C  the rest    : hub roots; chunks of their rows are spread over the
+//                    whole grid and count into dense per-root tables in HBM
+// An LDS hash slot packs (w+1) in the high bits and the count in the low
+// `cb` bits; roots with r_lo >= 2^cb go to bucket C.
+#include <hip/hip_runtime.h>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include <algorithm>
+#include <cstdio>
+#include <vector>
+
+#include "bt_common.h"
+#include "bt_internal.h"
+
+using namespace bt;
+
+namespace {
+
+constexpr uint32_t kWA = 1024;       // bucket A wedge limit
+constexpr uint32_t kCapA = 2048;     // bucket A hash slots
+constexpr uint32_t kWB = 8192;       // bucket B wedge limit
+constexpr uint32_t kCapB = 16384;    // bucket B hash slots
+constexpr int kBlockB = 256;
+constexpr int kBlockC = 256;
+
+// ------------------------------------------------------------------ prep ---
+
+__global__ void k_rank_keys(uint32_t n, const uint32_t* __restrict__ deg, uint64_t* __restrict__ keys) {
+    uint32_t const x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < n) keys[x] = (uint64_t(deg[x]) << 32) | x;
+}
+
+__global__ void k_rank_scatter(uint32_t n, const uint64_t* __restrict__ sorted, uint32_t* __restrict__ rank,
+                               uint32_t* __restrict__ r_deg) {
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        uint64_t const k = sorted[i];
+        rank[uint32_t(k)] = i;
+        r_deg[i] = uint32_t(k >> 32);
+    }
+}
+
+__global__ void k_set_last(uint32_t* __restrict__ a, uint32_t idx, uint32_t value) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) a[idx] = value;
+}
+
+__global__ void k_pair_keys(uint32_t m, const uint32_t* __restrict__ edges, const uint32_t* __restrict__ rank,
+                            uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    uint32_t const e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m) return;
+    uint64_t const ru = rank[edges[2ull * e]];
+    uint64_t const rv = rank[edges[2ull * e + 1]];
+    keys[2ull * e] = (ru << 32) | rv;
+    keys[2ull * e + 1] = (rv << 32) | ru;
+    vals[2ull * e] = e;
+    vals[2ull * e + 1] = e;
+}
+
+__global__ void k_pair_unpack(uint32_t cnt, const uint64_t* __restrict__ keys, uint32_t* __restrict__ r_nbr) {
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) r_nbr[i] = uint32_t(keys[i]);
+}
+
+__global__ void k_lo_deg(uint32_t n, const uint32_t* __restrict__ r_off, const uint32_t* __restrict__ r_nbr,
+                         uint32_t* __restrict__ r_lo) {
+    uint32_t const r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    uint32_t const b = r_off[r];
+    r_lo[r] = lower_bound_u32(r_nbr + b, r_off[r + 1] - b, r);
+}
+
+// r_p[i] for entry i = (r, v) of row r: #entries of row v below r, when v < r.
+__global__ void k_prefix_p(uint32_t cnt, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ r_off,
+                           const uint32_t* __restrict__ r_nbr, uint32_t* __restrict__ r_p) {
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    uint32_t const r = uint32_t(keys[i] >> 32);
+    uint32_t const v = r_nbr[i];
+    uint32_t p = 0;
+    if (v < r) {
+        uint32_t const b = r_off[v];
+        p = lower_bound_u32(r_nbr + b, r_off[v + 1] - b, r);
+    }
+    r_p[i] = p;
+}
+
+// One wavefront per root: W_r = sum of r_p over the row prefix.
+__global__ void k_root_w(uint32_t n, const uint32_t* __restrict__ r_off, const uint32_t* __restrict__ r_lo,
+                         const uint32_t* __restrict__ r_p, uint64_t* __restrict__ r_w, uint64_t* __restrict__ ctl) {
+    uint32_t const r = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    int const lane = threadIdx.x & (kWave - 1);
+    if (r >= n) return;
+    uint32_t const b = r_off[r], lo = r_lo[r];
+    uint64_t s = 0;
+    for (uint32_t i = lane; i < lo; i += kWave) s += r_p[b + i];
+    s = wave_sum_u64(s);
+    if (lane == 0) {
+        r_w[r] = s;
+        if (s) atomic_add_u64(&ctl[3], s);
+    }
+}
+
+__global__ void k_classify(uint32_t n, const uint64_t* __restrict__ r_w, const uint32_t* __restrict__ r_lo,
+                           uint32_t lo_limit, uint32_t* __restrict__ la, uint32_t* __restrict__ lb,
+                           uint32_t* __restrict__ lc, uint64_t* __restrict__ ctl) {
+    uint32_t const r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    uint64_t const w = r_w[r];
+    if (w == 0) return;
+    uint32_t const lo = r_lo[r];
+    if (lo < lo_limit && w <= kWA) {
+        la[atomicAdd(reinterpret_cast<unsigned long long*>(&ctl[0]), 1ull)] = r;
+    } else if (lo < lo_limit && w <= kWB) {
+        lb[atomicAdd(reinterpret_cast<unsigned long long*>(&ctl[1]), 1ull)] = r;
+    } else {
+        lc[atomicAdd(reinterpret_cast<unsigned long long*>(&ctl[2]), 1ull)] = r;
+    }
+}
+
+// --------------------------------------------------------------- counting ---
+
+// Packed LDS hash: slot = ((w+1) << cb) | count, 0 = empty.
+struct LdsHash {
+    uint32_t* tab;
+    uint32_t mask;   // capacity - 1
+    int shift;       // 32 - log2(capacity)
+    int cb;
+    __device__ __forceinline__ uint32_t slot0(uint32_t w) const { return (w * 2654435761u) >> shift; }
+    // increments the count of w, returns the count before the increment
+    // The table holds at most half its capacity of keys (distinct far ends
+    // <= wedges of the root <= capacity / 2), so probing ends; the probe cap
+    // only guards against a broken invariant.
+    __device__ __forceinline__ uint32_t inc(uint32_t w) const {
+        uint32_t const key = w + 1;
+        uint32_t s = slot0(w);
+        for (uint32_t probe = 0; probe <= mask; probe++) {
+            uint32_t cur = tab[s];
+            if (cur == 0) {
+                uint32_t const prev = atomicCAS(&tab[s], 0u, (key << cb) | 1u);
+                if (prev == 0) return 0;
+                cur = prev;
+            }
+            if ((cur >> cb) == key) {
+                uint32_t const old = atomicAdd(&tab[s], 1u);
+                return old & ((1u << cb) - 1u);
+            }
+            s = (s + 1) & mask;
+        }
+        return 0;
+    }
+    __device__ __forceinline__ uint32_t get(uint32_t w) const {
+        uint32_t const key = w + 1;
+        uint32_t s = slot0(w);
+        for (uint32_t probe = 0; probe <= mask; probe++) {
+            uint32_t const cur = tab[s];
+            if ((cur >> cb) == key) return cur & ((1u << cb) - 1u);
+            if (cur == 0) return 0;
+            s = (s + 1) & mask;
+        }
+        return 0;
+    }
+};
+
+struct DenseTable {
+    uint32_t* tab;
+    __device__ __forceinline__ uint32_t inc(uint32_t w) const { return atomicAdd(&tab[w], 1u); }
+    __device__ __forceinline__ uint32_t get(uint32_t w) const { return tab[w]; }
+};
+
+struct RowView {
+    const uint32_t* __restrict__ r_off;
+    const uint32_t* __restrict__ r_nbr;
+    const uint32_t* __restrict__ r_eid;
+    const uint32_t* __restrict__ r_p;
+};
+
+// Expands the wedges of entries [i0, i1) of root row (all rooted at r) in
+// chunks of BLOCK entries. PASS 1 counts far ends into the table and returns
+// the sum of the old counts (the butterflies); PASS 2 adds cnt-1 to the
+// supports of v-w (one atomic per wedge) and u-v (aggregated per entry).
+template <int BLOCK, int PASS, class Table>
+__device__ __forceinline__ uint64_t expand_entries(const RowView& g, uint32_t i0, uint32_t i1, const Table& tab,
+                                                   uint64_t* __restrict__ sup, uint32_t* s_start, uint32_t* s_pos,
+                                                   uint32_t* s_part, uint64_t* s_acc) {
+    uint64_t tot = 0;
+    int const tid = threadIdx.x;
+    for (uint32_t base = i0; base < i1; base += BLOCK) {
+        uint32_t const idx = base + tid;
+        bool const valid = idx < i1;
+        uint32_t const p = valid ? g.r_p[idx] : 0;
+        uint32_t const v = valid ? g.r_nbr[idx] : 0;
+        uint32_t chunk = 0;
+        uint32_t const excl = block_excl_scan<BLOCK>(p, s_part, chunk);
+        s_start[tid] = excl;
+        s_pos[tid] = valid ? g.r_off[v] : 0;
+        if (PASS == 2) s_acc[tid] = 0;
+        __syncthreads();
+        for (uint32_t t = tid; t < chunk; t += BLOCK) {
+            int const j = upper_index(s_start, BLOCK, t);
+            uint32_t const pos = s_pos[j] + (t - s_start[j]);
+            uint32_t const w = g.r_nbr[pos];
+            if (PASS == 1) {
+                tot += tab.inc(w);
+            } else {
+                uint32_t const c = tab.get(w);
+                if (c > 1) {
+                    atomic_add_u64(&sup[g.r_eid[pos]], uint64_t(c - 1));
+                    atomicAdd(reinterpret_cast<unsigned long long*>(&s_acc[j]), (unsigned long long)(c - 1));
+                }
+            }
+        }
+        __syncthreads();
+        if (PASS == 2) {
+            if (valid && s_acc[tid]) atomic_add_u64(&sup[g.r_eid[idx]], s_acc[tid]);
+            __syncthreads();
+        }
+    }
+    return tot;
+}
+
+// Buckets A and B: one root per workgroup iteration, table in LDS.
+template <int BLOCK, uint32_t CAP>
+__global__ __launch_bounds__(BLOCK) void k_count_lds(const uint32_t* __restrict__ roots, uint32_t nroots,
+                                                     RowView g, const uint32_t* __restrict__ r_lo,
+                                                     uint64_t* __restrict__ sup, uint64_t* __restrict__ total,
+                                                     int cb, int per_edge) {
+    __shared__ uint32_t tab[CAP];
+    __shared__ uint32_t s_start[BLOCK];
+    __shared__ uint32_t s_pos[BLOCK];
+    __shared__ uint64_t s_acc[BLOCK];
+    __shared__ uint32_t s_part[BLOCK / kWave + 1];
+    __shared__ uint64_t s_red[BLOCK / kWave + 1];
+    constexpr int kLog = __builtin_ctz(CAP);
+    LdsHash h{tab, CAP - 1, 32 - kLog, cb};
+    uint64_t tot = 0;
+    for (uint32_t ri = blockIdx.x; ri < nroots; ri += gridDim.x) {
+        uint32_t const r = roots[ri];
+        uint32_t const b = g.r_off[r], lo = r_lo[r];
+        for (uint32_t i = threadIdx.x; i < CAP; i += BLOCK) tab[i] = 0;
+        __syncthreads();
+        tot += expand_entries<BLOCK, 1>(g, b, b + lo, h, sup, s_start, s_pos, s_part, s_acc);
+        if (per_edge) expand_entries<BLOCK, 2>(g, b, b + lo, h, sup, s_start, s_pos, s_part, s_acc);
+        __syncthreads();
+    }
+    uint64_t const s = block_sum_u64<BLOCK>(tot, s_red);
+    if (threadIdx.x == 0 && s) atomic_add_u64(total, s);
+}
+
+// Bucket C: item = (root r, entry range [i0, i1), table offset); the dense
+// table of r has r slots (far ends have rank < r).
+struct HubItem {
+    uint32_t r, i0, i1, pad;
+    uint64_t toff;
+};
+
+template <int BLOCK, int PASS>
+__global__ __launch_bounds__(BLOCK) void k_count_dense(const HubItem* __restrict__ items, uint32_t nitems,
+                                                       RowView g, uint32_t* __restrict__ pool,
+                                                       uint64_t* __restrict__ sup, uint64_t* __restrict__ total) {
+    __shared__ uint32_t s_start[BLOCK];
+    __shared__ uint32_t s_pos[BLOCK];
+    __shared__ uint64_t s_acc[BLOCK];
+    __shared__ uint32_t s_part[BLOCK / kWave + 1];
+    __shared__ uint64_t s_red[BLOCK / kWave + 1];
+    uint64_t tot = 0;
+    for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+        HubItem const h = items[it];
+        DenseTable t{pool + h.toff};
+        tot += expand_entries<BLOCK, PASS>(g, h.i0, h.i1, t, sup, s_start, s_pos, s_part, s_acc);
+    }
+    if (PASS == 1) {
+        uint64_t const s = block_sum_u64<BLOCK>(tot, s_red);
+        if (threadIdx.x == 0 && s) atomic_add_u64(total, s);
+    }
+}
+
+__global__ void k_max_u64(uint32_t m, const uint64_t* __restrict__ a, uint64_t* __restrict__ out) {
+    uint64_t mx = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
+        mx = a[i] > mx ? a[i] : mx;
+    mx = wave_max_u64(mx);
+    if ((threadIdx.x & (kWave - 1)) == 0 && mx) atomic_max_u64(out, mx);
+}
+
+int bits_for(uint64_t x) {
+    int b = 0;
+    while (b < 64 && (x >> b)) b++;
+    return b;
+}
+
+}  // namespace
+
+int bt_prep_priority(bt_dgraph* dg) {
+    bt_ctx* c = dg->ctx;
+    hipStream_t s = c->stream;
+    uint32_t const n = dg->n, m = dg->m;
+    if (n == 0) return BT_OK;
+    constexpr unsigned T = 256;
+    int const nb = bits_for(n);
+    // temporary storage for the exact calls below
+    size_t t1 = 0, t2 = 0, t3 = 0;
+    BT_HIP_TRY(rocprim::radix_sort_keys(nullptr, t1, dg->r_key, dg->r_key2, n, 0, 64, s));
+    if (m)
+        BT_HIP_TRY(rocprim::radix_sort_pairs(nullptr, t2, dg->r_key, dg->r_key2, dg->r_val2, dg->r_eid,
+                                             2 * size_t(m), 0, 32 + nb, s));
+    BT_HIP_TRY(rocprim::exclusive_scan(nullptr, t3, dg->list0, dg->r_off, 0u, n, rocprim::plus<uint32_t>(), s));
+    BT_TRY(bt_buf_reserve(c->sort_tmp, std::max(t1, std::max(t2, t3))));
+    // rank = position in (degree, id) order
+    k_rank_keys<<<grid_for(n, T, 1u << 30), T, 0, s>>>(n, dg->deg, dg->r_key);
+    BT_HIP_TRY(rocprim::radix_sort_keys(c->sort_tmp.ptr, t1, dg->r_key, dg->r_key2, n, 0, 64, s));
+    k_rank_scatter<<<grid_for(n, T, 1u << 30), T, 0, s>>>(n, dg->r_key2, dg->rank, dg->list0);
+    BT_HIP_TRY(rocprim::exclusive_scan(c->sort_tmp.ptr, t3, dg->list0, dg->r_off, 0u, n,
+                                       rocprim::plus<uint32_t>(), s));
+    k_set_last<<<1, 64, 0, s>>>(dg->r_off, n, 2 * m);
+    if (m) {
+        // rows of the relabelled CSR: sort directed pairs by (rank, rank)
+        k_pair_keys<<<grid_for(m, T, 1u << 30), T, 0, s>>>(m, dg->edges, dg->rank, dg->r_key, dg->r_val2);
+        BT_HIP_TRY(rocprim::radix_sort_pairs(c->sort_tmp.ptr, t2, dg->r_key, dg->r_key2, dg->r_val2, dg->r_eid,
+                                             2 * size_t(m), 0, 32 + nb, s));
+        k_pair_unpack<<<grid_for(2ull * m, T, 1u << 30), T, 0, s>>>(2 * m, dg->r_key2, dg->r_nbr);
+    }
+    k_lo_deg<<<grid_for(n, T, 1u << 30), T, 0, s>>>(n, dg->r_off, dg->r_nbr, dg->r_lo);
+    if (m) k_prefix_p<<<grid_for(2ull * m, T, 1u << 30), T, 0, s>>>(2 * m, dg->r_key2, dg->r_off, dg->r_nbr, dg->r_p);
+    BT_HIP_TRY(hipGetLastError());
+    return BT_OK;
+}
+
+int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total_out, uint64_t* smax_out) {
+    bt_ctx* c = dg->ctx;
+    hipStream_t s = c->stream;
+    uint32_t const n = dg->n, m = dg->m;
+    c->last = bt_timings{};
+    bt_timer tm(c);
+    BT_TRY(tm.start(0));
+    dg->has_support = false;
+    if (m) BT_HIP_TRY(hipMemsetAsync(dg->sup, 0, sizeof(uint64_t) * m, s));
+    BT_HIP_TRY(hipMemsetAsync(dg->ctl, 0, sizeof(uint64_t) * 64, s));
+    if (m == 0 || n == 0) {
+        BT_TRY(tm.start(1));
+        BT_HIP_TRY(hipStreamSynchronize(s));
+        if (total_out) *total_out = 0;
+        if (smax_out) *smax_out = 0;
+        dg->has_support = per_edge;
+        c->last.total_ms = tm.ms(0, 1);
+        return BT_OK;
+    }
+    BT_TRY(bt_prep_priority(dg));
+    BT_TRY(tm.start(1));
+    constexpr unsigned T = 256;
+    k_root_w<<<grid_for(uint64_t(n) * kWave, T, 1u << 30), T, 0, s>>>(n, dg->r_off, dg->r_lo, dg->r_p, dg->r_w,
+                                                                        dg->ctl);
+    int const kb = bits_for(uint64_t(n));  // bits of w+1 <= n
+    int const cb = 32 - kb;
+    uint32_t const lo_limit = cb >= 31 ? 0x7FFFFFFFu : (1u << cb);
+    uint32_t* la = dg->list0;
+    uint32_t* lb = dg->list1;
+    uint32_t* lc = dg->r_val2;
+    k_classify<<<grid_for(n, T, 1u << 30), T, 0, s>>>(n, dg->r_w, dg->r_lo, cb > 0 ? lo_limit : 0, la, lb, lc,
+                                                      dg->ctl);
+    uint64_t cnts[4];
+    BT_TRY(bt_read_ctl(dg, 0, 4, cnts));
+    uint64_t const na = cnts[0], nb = cnts[1], nc = cnts[2];
+    c->last.wedges_count = cnts[3];
+    RowView g{dg->r_off, dg->r_nbr, dg->r_eid, dg->r_p};
+    uint64_t* total = dg->ctl + 8;
+    unsigned const max_grid = unsigned(c->cu_count) * 16;
+    BT_TRY(tm.start(2));
+    if (na)
+        k_count_lds<kWave, kCapA><<<grid_for(na, 1, max_grid * 4), kWave, 0, s>>>(la, uint32_t(na), g, dg->r_lo,
+                                                                               dg->sup, total, cb, per_edge);
+    if (nb)
+        k_count_lds<kBlockB, kCapB><<<grid_for(nb, 1, max_grid), kBlockB, 0, s>>>(lb, uint32_t(nb), g, dg->r_lo,
+                                                                               dg->sup, total, cb, per_edge);
+    BT_HIP_TRY(hipGetLastError());
+    if (nc) {
+        // hub roots: build (root, entry chunk) items on the host, batch them
+        // so that their dense tables fit the pool
+        std::vector<uint32_t> hroots(nc), hlo(nc), hoff(nc);
+        BT_HIP_TRY(hipMemcpyAsync(hroots.data(), lc, sizeof(uint32_t) * nc, hipMemcpyDeviceToHost, s));
+        BT_HIP_TRY(hipStreamSynchronize(s));
+        std::sort(hroots.begin(), hroots.end());
+        std::vector<uint32_t> all_lo(n), all_off(n + 1);
+        BT_HIP_TRY(hipMemcpyAsync(all_lo.data(), dg->r_lo, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+        BT_HIP_TRY(hipMemcpyAsync(all_off.data(), dg->r_off, sizeof(uint32_t) * (n + 1), hipMemcpyDeviceToHost, s));
+        BT_HIP_TRY(hipStreamSynchronize(s));
+        size_t const pool_entries_max = size_t(256) << 20;  // 1 GiB of u32 counters
+        size_t max_table = 0;
+        for (uint32_t r : hroots) max_table = std::max<size_t>(max_table, r);
+        size_t const pool_entries = std::max(max_table, std::min<size_t>(pool_entries_max, size_t(n) * nc));
+        BT_TRY(bt_buf_reserve(c->pool, pool_entries * sizeof(uint32_t)));
+        uint32_t* pool = static_cast<uint32_t*>(c->pool.ptr);
+        HubItem* ditems = reinterpret_cast<HubItem*>(dg->r_key);  // scratch: 2m+n u64 >= items
+        size_t const items_cap = (size_t(2) * m + n) * sizeof(uint64_t) / sizeof(HubItem);
+        size_t k = 0;
+        while (k < nc) {
+            std::vector<HubItem> items;
+            size_t used = 0;
+            size_t k2 = k;
+            while (k2 < nc && used + hroots[k2] <= pool_entries) {
+                uint32_t const r = hroots[k2];
+                uint32_t const b = all_off[r], lo = all_lo[r];
+                for (uint32_t i = 0; i < lo; i += kBlockC)
+                    items.push_back(HubItem{r, b + i, b + std::min<uint32_t>(lo, i + kBlockC), 0, used});
+                used += r;
+                k2++;
+            }
+            if (k2 == k || items.size() > items_cap) return BT_ERR_INTERNAL;
+            BT_HIP_TRY(hipMemsetAsync(pool, 0, used * sizeof(uint32_t), s));
+            BT_HIP_TRY(hipMemcpyAsync(ditems, items.data(), items.size() * sizeof(HubItem), hipMemcpyHostToDevice, s));
+            unsigned const gC = grid_for(items.size(), 1, max_grid);
+            k_count_dense<kBlockC, 1><<<gC, kBlockC, 0, s>>>(ditems, uint32_t(items.size()), g, pool, dg->sup, total);
+            if (per_edge)
+                k_count_dense<kBlockC, 2><<<gC, kBlockC, 0, s>>>(ditems, uint32_t(items.size()), g, pool, dg->sup,
+                                                                 total);
+            BT_HIP_TRY(hipGetLastError());
+            // the host vector `items` is reused only after this batch's copy
+            BT_HIP_TRY(hipStreamSynchronize(s));
+            k = k2;
+        }
+    }
+    BT_TRY(tm.start(3));
+    if (per_edge) {
+        uint64_t* smax = dg->ctl + 9;
+        k_max_u64<<<grid_for(m, T, unsigned(c->cu_count) * 8), T, 0, s>>>(m, dg->sup, smax);
+        BT_HIP_TRY(hipGetLastError());
+    }
+    BT_TRY(tm.start(4));
+    uint64_t res[2];
+    BT_TRY(bt_read_ctl(dg, 8, 2, res));
+    if (total_out) *total_out = res[0];
+    if (smax_out) *smax_out = res[1];
+    dg->has_support = per_edge;
+    c->last.prep_ms = tm.ms(0, 1);
+    c->last.count_kernel_ms = tm.ms(2, 3);
+    c->last.count_ms = tm.ms(1, 4);
+    c->last.total_ms = tm.ms(0, 4);
+    return BT_OK;
+}

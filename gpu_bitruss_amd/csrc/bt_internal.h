@@ -1,0 +1,99 @@
+// Internal declarations shared by the HIP translation units of
+// libbitruss_hip.so. Not part of the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "bitruss_hip.h"
+
+#define BT_HIP_TRY(expr)                                   \
+    do {                                                   \
+        hipError_t bt_e_ = (expr);                         \
+        if (bt_e_ != hipSuccess) {                         \
+            bt_set_hip_error(bt_e_, #expr, __FILE__, __LINE__); \
+            return BT_ERR_DEVICE;                          \
+        }                                                  \
+    } while (0)
+
+#define BT_TRY(expr)                     \
+    do {                                 \
+        int bt_s_ = (expr);              \
+        if (bt_s_ != BT_OK) return bt_s_; \
+    } while (0)
+
+void bt_set_hip_error(hipError_t e, const char* what, const char* file, int line);
+
+// A device buffer that grows on demand and is reused between calls.
+struct bt_buf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+int bt_buf_reserve(bt_buf& b, size_t bytes);
+void bt_buf_free(bt_buf& b);
+
+struct bt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    size_t lds_per_block = 0;
+    bt_timings last{};
+    hipEvent_t ev[8]{};
+    // scratch shared by all graphs of the context
+    bt_buf sort_tmp;   // rocprim temporary storage
+    bt_buf pool;       // dense per-root count tables for hub roots
+    bt_buf pinned_dummy;
+    uint64_t* h_pinned = nullptr;  // small pinned host buffer for read-backs
+};
+
+// Device-resident graph plus the per-graph scratch of the algorithms.
+struct bt_dgraph {
+    bt_ctx* ctx = nullptr;
+    uint32_t u_num = 0, l_num = 0, n = 0, m = 0;
+    // the uploaded reference CSR (original ids)
+    uint32_t* nbr = nullptr;    // 2m
+    uint32_t* off = nullptr;    // n+1
+    uint32_t* deg = nullptr;    // n
+    uint32_t* edges = nullptr;  // 2m
+    uint32_t* eids = nullptr;   // 2m
+    // priority-relabelled CSR (rank = position in (degree, id) order)
+    uint32_t* rank = nullptr;   // n   : original id -> rank
+    uint32_t* r_off = nullptr;  // n+1
+    uint32_t* r_nbr = nullptr;  // 2m  : neighbour ranks, ascending per row
+    uint32_t* r_eid = nullptr;  // 2m
+    uint32_t* r_lo = nullptr;   // n   : neighbours of lower rank (a row prefix)
+    uint32_t* r_p = nullptr;    // 2m  : per lower neighbour v of r, #N(v) below r
+    uint64_t* r_w = nullptr;    // n   : priority wedges rooted at r
+    uint64_t* sup = nullptr;    // m   : ebfc supports (then msp working supports)
+    uint64_t* res = nullptr;    // m   : hidx / msp results
+    uint64_t* aux = nullptr;    // m   : second hidx buffer
+    uint8_t* state = nullptr;   // m   : msp edge state
+    uint32_t* list0 = nullptr;  // m   : frontier buffers / root lists
+    uint32_t* list1 = nullptr;  // m
+    uint64_t* ctl = nullptr;    // 64 control words
+    uint64_t* r_key = nullptr;  // 2m  : sort keys (prep only)
+    uint64_t* r_key2 = nullptr; // 2m
+    uint32_t* r_val2 = nullptr; // 2m
+    bool has_support = false;   // sup holds the ebfc result of this graph
+};
+
+// per-call helpers
+struct bt_timer {
+    bt_ctx* ctx;
+    explicit bt_timer(bt_ctx* c) : ctx(c) {}
+    int start(int slot) { return hipEventRecord(ctx->ev[slot], ctx->stream) == hipSuccess ? BT_OK : BT_ERR_DEVICE; }
+    double ms(int a, int b);
+};
+
+// Algorithm entry points implemented in the .hip files (inputs checked by
+// the C-ABI layer).
+int bt_prep_priority(bt_dgraph* dg);
+int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total, uint64_t* support_max);
+int bt_msp_impl(bt_dgraph* dg, uint32_t* levels);
+int bt_hidx_impl(bt_dgraph* dg, uint32_t* rounds);
+
+// Read back a few control words (synchronises the stream).
+int bt_read_ctl(bt_dgraph* dg, int first, int count, uint64_t* out);

@@ -1,0 +1,284 @@
+// Runtime layer of libbitruss_hip.so: contexts, resident device graphs and
+// the C-ABI entry points (argument checks, timing, result copies).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "bt_internal.h"
+
+namespace {
+thread_local char g_last_hip_error[512] = "";
+}
+
+void bt_set_hip_error(hipError_t e, const char* what, const char* file, int line) {
+    std::snprintf(g_last_hip_error, sizeof(g_last_hip_error), "%s: %s (%s:%d)",
+                  hipGetErrorString(e), what, file, line);
+    std::fprintf(stderr, "[bitruss_hip] HIP error %s\n", g_last_hip_error);
+}
+
+int bt_buf_reserve(bt_buf& b, size_t bytes) {
+    if (bytes <= b.bytes && b.ptr) return BT_OK;
+    if (b.ptr) {
+        BT_HIP_TRY(hipFree(b.ptr));
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    size_t const want = bytes ? bytes : 256;
+    BT_HIP_TRY(hipMalloc(&b.ptr, want));
+    b.bytes = want;
+    return BT_OK;
+}
+
+void bt_buf_free(bt_buf& b) {
+    if (b.ptr) (void)hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+}
+
+double bt_timer::ms(int a, int b) {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, ctx->ev[a], ctx->ev[b]) != hipSuccess) return -1.0;
+    return double(t);
+}
+
+int bt_read_ctl(bt_dgraph* dg, int first, int count, uint64_t* out) {
+    bt_ctx* c = dg->ctx;
+    BT_HIP_TRY(hipMemcpyAsync(c->h_pinned, dg->ctl + first, sizeof(uint64_t) * count,
+                              hipMemcpyDeviceToHost, c->stream));
+    BT_HIP_TRY(hipStreamSynchronize(c->stream));
+    std::memcpy(out, c->h_pinned, sizeof(uint64_t) * count);
+    return BT_OK;
+}
+
+namespace {
+
+template <typename T>
+int dalloc(T** p, size_t count) {
+    BT_HIP_TRY(hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (count ? count : 1)));
+    return BT_OK;
+}
+
+int check_ctx(bt_ctx* ctx) {
+    if (!ctx) return BT_ERR_INVALID;
+    BT_HIP_TRY(hipSetDevice(ctx->device));
+    return BT_OK;
+}
+
+int check_graph(bt_ctx* ctx, bt_dgraph* dg) {
+    if (!ctx || !dg || dg->ctx != ctx) return BT_ERR_INVALID;
+    return check_ctx(ctx);
+}
+
+int copy_out(bt_dgraph* dg, const uint64_t* dsrc, uint64_t* host) {
+    if (!host || dg->m == 0) return BT_OK;
+    BT_HIP_TRY(hipMemcpyAsync(host, dsrc, sizeof(uint64_t) * dg->m, hipMemcpyDeviceToHost,
+                              dg->ctx->stream));
+    BT_HIP_TRY(hipStreamSynchronize(dg->ctx->stream));
+    return BT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bt_device_count(int* count) {
+    if (!count) return BT_ERR_INVALID;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *count = c;
+    return BT_OK;
+}
+
+int bt_device_info(int device, bt_device_props* out) {
+    if (!out) return BT_ERR_INVALID;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return BT_ERR_NO_DEVICE;
+    hipDeviceProp_t p{};
+    BT_HIP_TRY(hipGetDeviceProperties(&p, device));
+    std::memset(out, 0, sizeof(*out));
+    std::snprintf(out->name, sizeof(out->name), "%s", p.name);
+    std::snprintf(out->arch, sizeof(out->arch), "%s", p.gcnArchName);
+    out->memory_bus_width_bits = p.memoryBusWidth;
+    out->memory_clock_khz = p.memoryClockRate;
+    out->total_global_mem = p.totalGlobalMem;
+    out->multiprocessor_count = p.multiProcessorCount;
+    out->shared_mem_per_block = p.sharedMemPerBlock;
+    out->max_threads_per_multiprocessor = p.maxThreadsPerMultiProcessor;
+    out->warp_size = p.warpSize;
+    out->l2_cache_bytes = p.l2CacheSize;
+    return BT_OK;
+}
+
+int bt_ctx_create(int device, bt_ctx** out) {
+    if (!out) return BT_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return BT_ERR_NO_DEVICE;
+    if (device < 0 || device >= count) return BT_ERR_NO_DEVICE;
+    BT_HIP_TRY(hipSetDevice(device));
+    bt_ctx* c = new (std::nothrow) bt_ctx();
+    if (!c) return BT_ERR_NOMEM;
+    c->device = device;
+    hipDeviceProp_t prop{};
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return BT_ERR_DEVICE;
+    }
+    c->cu_count = prop.multiProcessorCount;
+    c->lds_per_block = prop.sharedMemPerBlock;
+    for (auto& e : c->ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            bt_ctx_destroy(c);
+            return BT_ERR_DEVICE;
+        }
+    }
+    if (hipHostMalloc(reinterpret_cast<void**>(&c->h_pinned), 64 * sizeof(uint64_t)) != hipSuccess) {
+        bt_ctx_destroy(c);
+        return BT_ERR_DEVICE;
+    }
+    *out = c;
+    return BT_OK;
+}
+
+int bt_ctx_destroy(bt_ctx* c) {
+    if (!c) return BT_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    bt_buf_free(c->sort_tmp);
+    bt_buf_free(c->pool);
+    bt_buf_free(c->pinned_dummy);
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return BT_OK;
+}
+
+int bt_ctx_last_timings(const bt_ctx* ctx, bt_timings* out) {
+    if (!ctx || !out) return BT_ERR_INVALID;
+    *out = ctx->last;
+    return BT_OK;
+}
+
+int bt_graph_free(bt_dgraph* dg) {
+    if (!dg) return BT_ERR_INVALID;
+    if (dg->ctx) {
+        (void)hipSetDevice(dg->ctx->device);
+        (void)hipStreamSynchronize(dg->ctx->stream);
+    }
+    void* ptrs[] = {dg->nbr,  dg->off,   dg->deg,   dg->edges, dg->eids,  dg->rank,
+                    dg->r_off, dg->r_nbr, dg->r_eid, dg->r_lo,  dg->r_p,   dg->r_w,
+                    dg->sup,  dg->res,   dg->aux,   dg->state, dg->list0, dg->list1,
+                    dg->ctl,  dg->r_key, dg->r_key2, dg->r_val2};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete dg;
+    return BT_OK;
+}
+
+int bt_graph_upload(bt_ctx* ctx, const bt_host_graph* g, bt_dgraph** out) {
+    if (!out) return BT_ERR_INVALID;
+    *out = nullptr;
+    BT_TRY(check_ctx(ctx));
+    if (!g) return BT_ERR_INVALID;
+    // host-side validation: the kernels index with these arrays
+    BT_TRY(bt_graph_check(g));
+    bt_dgraph* dg = new (std::nothrow) bt_dgraph();
+    if (!dg) return BT_ERR_NOMEM;
+    dg->ctx = ctx;
+    dg->u_num = g->u_num;
+    dg->l_num = g->l_num;
+    dg->n = g->n;
+    dg->m = g->m;
+    size_t const n = g->n, m = g->m;
+    int st = BT_OK;
+    auto A = [&](int s) {
+        if (st == BT_OK) st = s;
+    };
+    A(dalloc(&dg->nbr, 2 * m));
+    A(dalloc(&dg->off, n + 1));
+    A(dalloc(&dg->deg, n));
+    A(dalloc(&dg->edges, 2 * m));
+    A(dalloc(&dg->eids, 2 * m));
+    A(dalloc(&dg->rank, n));
+    A(dalloc(&dg->r_off, n + 1));
+    A(dalloc(&dg->r_nbr, 2 * m));
+    A(dalloc(&dg->r_eid, 2 * m));
+    A(dalloc(&dg->r_lo, n));
+    A(dalloc(&dg->r_p, 2 * m));
+    A(dalloc(&dg->r_w, n + 1));
+    A(dalloc(&dg->sup, m));
+    A(dalloc(&dg->res, m));
+    A(dalloc(&dg->aux, m));
+    A(dalloc(&dg->state, m));
+    A(dalloc(&dg->list0, m + n + 1));
+    A(dalloc(&dg->list1, m + n + 1));
+    A(dalloc(&dg->ctl, 64));
+    A(dalloc(&dg->r_key, 2 * m + n));
+    A(dalloc(&dg->r_key2, 2 * m + n));
+    A(dalloc(&dg->r_val2, 2 * m + n));
+    if (st != BT_OK) {
+        bt_graph_free(dg);
+        return st == BT_ERR_DEVICE ? BT_ERR_BUDGET : st;
+    }
+    hipStream_t s = ctx->stream;
+    auto H2D = [&](void* d, const void* h, size_t bytes) {
+        if (st != BT_OK || bytes == 0) return;
+        if (hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s) != hipSuccess) st = BT_ERR_DEVICE;
+    };
+    H2D(dg->nbr, g->neighbors, 8 * m);
+    H2D(dg->off, g->offsets, 4 * (n + 1));
+    H2D(dg->deg, g->degrees, 4 * n);
+    H2D(dg->edges, g->edges, 8 * m);
+    H2D(dg->eids, g->edge_ids, 8 * m);
+    if (st == BT_OK && hipStreamSynchronize(s) != hipSuccess) st = BT_ERR_DEVICE;
+    if (st != BT_OK) {
+        bt_graph_free(dg);
+        return st;
+    }
+    *out = dg;
+    return BT_OK;
+}
+
+int bt_bfc(bt_ctx* ctx, bt_dgraph* dg, uint64_t* total) {
+    BT_TRY(check_graph(ctx, dg));
+    uint64_t t = 0;
+    BT_TRY(bt_count_impl(dg, false, &t, nullptr));
+    if (total) *total = t;
+    return BT_OK;
+}
+
+int bt_ebfc(bt_ctx* ctx, bt_dgraph* dg, uint64_t* support_out, uint64_t* total,
+            uint64_t* support_max) {
+    BT_TRY(check_graph(ctx, dg));
+    uint64_t t = 0, smax = 0;
+    BT_TRY(bt_count_impl(dg, true, &t, &smax));
+    BT_TRY(copy_out(dg, dg->sup, support_out));
+    if (total) *total = t;
+    if (support_max) *support_max = smax;
+    return BT_OK;
+}
+
+int bt_hidx(bt_ctx* ctx, bt_dgraph* dg, uint64_t* bitruss_out, uint32_t* rounds) {
+    BT_TRY(check_graph(ctx, dg));
+    uint32_t r = 0;
+    BT_TRY(bt_hidx_impl(dg, &r));
+    BT_TRY(copy_out(dg, dg->res, bitruss_out));
+    if (rounds) *rounds = r;
+    return BT_OK;
+}
+
+int bt_msp(bt_ctx* ctx, bt_dgraph* dg, uint64_t* bitruss_out, uint32_t* levels) {
+    BT_TRY(check_graph(ctx, dg));
+    uint32_t l = 0;
+    BT_TRY(bt_msp_impl(dg, &l));
+    BT_TRY(copy_out(dg, dg->res, bitruss_out));
+    if (levels) *levels = l;
+    return BT_OK;
+}
+
+}  // extern "C"

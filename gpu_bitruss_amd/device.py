@@ -1,0 +1,169 @@
+"""Device-side API: contexts, resident graphs and the reference's four
+algorithm entry points (src/bfc/bfc.cuh:17-23, src/bitruss/bitruss.cuh:17-24)
+on top of the C-ABI in include/bitruss_hip.h.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .graph import Graph
+
+
+def device_count() -> int:
+    c = C.c_int(0)
+    _lib.check("bt_device_count", _lib.lib().bt_device_count(C.byref(c)))
+    return c.value
+
+
+class Device:
+    """A bt_ctx: one HIP stream plus scratch on one GPU."""
+
+    def __init__(self, device: int = 0):
+        self._ctx = C.c_void_p()
+        _lib.check("bt_ctx_create", _lib.lib().bt_ctx_create(device, C.byref(self._ctx)))
+        self.device = device
+
+    @property
+    def handle(self) -> C.c_void_p:
+        if not self._ctx:
+            raise RuntimeError("device context is closed")
+        return self._ctx
+
+    def upload(self, g: Graph) -> "DeviceGraph":
+        return DeviceGraph(self, g)
+
+    def last_timings(self) -> dict:
+        t = _lib.BtTimings()
+        _lib.check("bt_ctx_last_timings", _lib.lib().bt_ctx_last_timings(self.handle, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def close(self) -> None:
+        if self._ctx:
+            _lib.lib().bt_ctx_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceGraph:
+    """A graph resident in device memory (bt_dgraph)."""
+
+    def __init__(self, dev: Device, g: Graph):
+        self.dev = dev
+        self.m = g.m
+        self._h = C.c_void_p()
+        hg = g.host_struct()
+        _lib.check("bt_graph_upload", _lib.lib().bt_graph_upload(dev.handle, C.byref(hg), C.byref(self._h)))
+
+    @property
+    def handle(self) -> C.c_void_p:
+        if not self._h:
+            raise RuntimeError("device graph is freed")
+        return self._h
+
+    def _out(self, copy: bool):
+        if not copy:
+            return None, None
+        a = np.zeros(max(self.m, 1), dtype=np.uint64)
+        return a, a.ctypes.data_as(_lib.u64p)
+
+    def bfc(self) -> int:
+        t = C.c_uint64()
+        _lib.check("bt_bfc", _lib.lib().bt_bfc(self.dev.handle, self.handle, C.byref(t)))
+        return t.value
+
+    def ebfc(self, copy: bool = True):
+        a, p = self._out(copy)
+        t, s = C.c_uint64(), C.c_uint64()
+        _lib.check("bt_ebfc", _lib.lib().bt_ebfc(self.dev.handle, self.handle, p, C.byref(t), C.byref(s)))
+        return (a[: self.m] if a is not None else None), t.value, s.value
+
+    def hidx(self, copy: bool = True):
+        a, p = self._out(copy)
+        r = C.c_uint32()
+        _lib.check("bt_hidx", _lib.lib().bt_hidx(self.dev.handle, self.handle, p, C.byref(r)))
+        return (a[: self.m] if a is not None else None), r.value
+
+    def msp(self, copy: bool = True):
+        a, p = self._out(copy)
+        lv = C.c_uint32()
+        _lib.check("bt_msp", _lib.lib().bt_msp(self.dev.handle, self.handle, p, C.byref(lv)))
+        return (a[: self.m] if a is not None else None), lv.value
+
+    def free(self) -> None:
+        if self._h:
+            _lib.lib().bt_graph_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+_default: Device | None = None
+
+
+def default_device() -> Device:
+    global _default
+    if _default is None:
+        _default = Device(0)
+    return _default
+
+
+# ---- the reference's free functions on a Graph ---------------------------
+
+def bfc_vpp(g: Graph, dev: Device | None = None) -> int:
+    """Total butterflies (bfc_vpp.cu:321-385); stored in g.butterflies."""
+    dg = (dev or default_device()).upload(g)
+    try:
+        g.butterflies = dg.bfc()
+    finally:
+        dg.free()
+    return g.butterflies
+
+
+def bfc_evpp(g: Graph, dev: Device | None = None) -> None:
+    """Per-edge supports into g.edge_support and g.support_max
+    (bfc_vpp.cu:233-319)."""
+    dg = (dev or default_device()).upload(g)
+    try:
+        g.edge_support, g.butterflies, g.support_max = dg.ebfc()
+    finally:
+        dg.free()
+
+
+def g_bitruss_hindex(g: Graph, dev: Device | None = None) -> int:
+    """h-index bitruss numbers into g.bitruss (bitruss_hidx.cu:179-256);
+    returns the number of rounds."""
+    dg = (dev or default_device()).upload(g)
+    try:
+        g.bitruss, rounds = dg.hidx()
+    finally:
+        dg.free()
+    return rounds
+
+
+def bitruss_msp(g: Graph, dev: Device | None = None) -> int:
+    """GBiD msp results into g.bitruss (bitruss_msp.cu:440-583); returns the
+    number of levels the reference logs."""
+    dg = (dev or default_device()).upload(g)
+    try:
+        g.bitruss, levels = dg.msp()
+    finally:
+        dg.free()
+    return levels

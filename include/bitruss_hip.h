@@ -1,0 +1,111 @@
+/*
+ * bitruss_hip.h — C-ABI of the MI355X (gfx950) bitruss library.
+ *
+ * The reference has no plugin or operator API: its boundary is a set of
+ * C++ free functions on a mutable Graph (src/bfc/bfc.cuh:17-23,
+ * src/bitruss/bitruss.cuh:17-24) called only from main.cu:96-129. Each
+ * entry point below replaces one of them; the reference's CUDA functions
+ * allocate, copy and free all device memory per call and exit() on error,
+ * while here the device graph stays resident between calls and every call
+ * returns a status (0 = BT_OK, text via bt_status_str in bt_graph.h).
+ *
+ * Results are bit-exact with the reference CPU semantics, widened to
+ * uint64 (the reference keeps supports in int32).
+ *
+ * Threading: one bt_ctx per host thread; calls on one context are
+ * serialised on its HIP stream. Contexts on the same device may be used
+ * concurrently from different threads.
+ */
+#ifndef BITRUSS_HIP_H
+#define BITRUSS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "bt_graph.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bt_ctx bt_ctx;       /* device, stream, scratch pool  */
+typedef struct bt_dgraph bt_dgraph; /* a graph resident on a device  */
+
+/* Per-call measurements of the last algorithm call on a context
+ * (HIP events on the context's stream). */
+typedef struct bt_timings {
+    double total_ms;               /* whole call, inputs resident            */
+    double prep_ms;                /* priority relabel of the CSR            */
+    double count_ms;               /* butterfly / support counting           */
+    double peel_ms;                /* msp peeling or hidx rounds             */
+    double peel_kernel_ms;         /* sum of the peel kernel durations       */
+    uint64_t peel_kernel_launches; /* launches of the peel kernel            */
+    double count_kernel_ms;        /* sum of the counting kernel durations   */
+    uint64_t count_kernel_launches;
+    uint64_t wedges_count;         /* priority wedges counted                */
+    uint64_t wedges_peel;          /* reserved                               */
+    uint64_t sub_rounds;           /* msp sub-rounds / hidx rounds           */
+    uint64_t levels_scanned;       /* msp levels visited                     */
+} bt_timings;
+
+/* Number of HIP devices (0 when none). Replaces cudaGetDeviceCount use
+ * in main.cu:54-55. */
+int bt_device_count(int* count);
+
+/* Device properties for `--device_info` (utility.h:127-146 prints the same
+ * table from cudaGetDeviceProperties). */
+typedef struct bt_device_props {
+    char name[256];
+    char arch[64];                 /* gcnArchName, e.g. gfx950:sramecc+:xnack- */
+    int memory_bus_width_bits;
+    int memory_clock_khz;
+    uint64_t total_global_mem;
+    int multiprocessor_count;
+    uint64_t shared_mem_per_block;
+    int max_threads_per_multiprocessor;
+    int warp_size;
+    int l2_cache_bytes;
+} bt_device_props;
+int bt_device_info(int device, bt_device_props* out);
+
+/* Create a context on `device` (main.cu:58-65 device selection): one
+ * HIP stream and a reusable scratch pool. */
+int bt_ctx_create(int device, bt_ctx** out);
+int bt_ctx_destroy(bt_ctx* ctx);
+int bt_ctx_last_timings(const bt_ctx* ctx, bt_timings* out);
+
+/* Copy a host CSR (the reference Graph arrays, graph.h:43-51) to the
+ * device and keep it resident. Replaces the per-call cudaMalloc/cudaMemcpy
+ * of bfc_vpp.cu:233-262, bitruss_hidx.cu:196-211, bitruss_msp.cu:447-465. */
+int bt_graph_upload(bt_ctx* ctx, const bt_host_graph* g, bt_dgraph** out);
+int bt_graph_free(bt_dgraph* dg);
+
+/* Total butterfly count. Replaces bfc_vpp (bfc_vpp.cu:321-385), whose
+ * total is only logged (bfc_vpp.cu:366). */
+int bt_bfc(bt_ctx* ctx, bt_dgraph* dg, uint64_t* total);
+
+/* Per-edge butterfly support. Replaces bfc_evpp (bfc_vpp.cu:233-319),
+ * which writes g->edge_support and g->support_max. support_out (m
+ * entries) may be NULL to keep the result on the device; total and
+ * support_max may be NULL. */
+int bt_ebfc(bt_ctx* ctx, bt_dgraph* dg, uint64_t* support_out, uint64_t* total,
+            uint64_t* support_max);
+
+/* h-index bitruss decomposition. Replaces g_bitruss_hindex
+ * (bitruss_hidx.cu:179-256), which writes g->bitruss. bitruss_out (m
+ * entries) may be NULL; rounds (may be NULL) receives the number of
+ * rounds run. */
+int bt_hidx(bt_ctx* ctx, bt_dgraph* dg, uint64_t* bitruss_out, uint32_t* rounds);
+
+/* GBiD multi-stage peeling with the reference's attribution rule.
+ * Replaces bitruss_msp (bitruss_msp.cu:440-583), which writes
+ * g->bitruss. bitruss_out (m entries) may be NULL; levels (may be NULL)
+ * receives the value the reference logs as "levels" (the largest
+ * bitruss number, 0 for a butterfly-free graph). */
+int bt_msp(bt_ctx* ctx, bt_dgraph* dg, uint64_t* bitruss_out, uint32_t* levels);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BITRUSS_HIP_H */

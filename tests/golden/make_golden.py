@@ -1,0 +1,63 @@
+"""Writes tests/golden/<workload>.json: digests of the CPU oracle's results
+on the named seeded workloads, so GPU tests can check full-size outputs
+without rerunning the slow CPU algorithms.
+
+    python tests/golden/make_golden.py small_uniform powerlaw_2m
+
+Each entry holds sha256 of the little-endian uint64 arrays (ebfc supports,
+msp results, hidx results), plus the totals and the oracle work counters.
+The hidx oracle is slow on powerlaw_2m (Jacobi rounds on 8 threads take
+tens of minutes); pass --skip-hidx to leave it out.
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import gpu_bitruss_amd as gb  # noqa: E402
+from oracle import oracle  # noqa: E402
+
+
+def digest(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<u8").tobytes()).hexdigest()
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("workloads", nargs="+")
+    ap.add_argument("--threads", type=int, default=os.cpu_count() or 4)
+    ap.add_argument("--skip-hidx", action="store_true")
+    args = ap.parse_args()
+    for wl in args.workloads:
+        g = gb.Graph.generate(wl)
+        sup, total, smax, st = oracle.ebfc(g, args.threads)
+        res, levels, st2 = oracle.msp(g, args.threads)
+        out = {
+            "workload": wl, "m": g.m, "n": g.n, "u_num": g.u_num, "l_num": g.l_num,
+            "neighbors_sha256": hashlib.sha256(g.neighbors.tobytes()).hexdigest(),
+            "butterflies": total, "support_max": smax, "ebfc_sha256": digest(sup),
+            "msp_levels": levels, "msp_sha256": digest(res),
+            "w_cnt": st["w_cnt"], "msp_stats": st2,
+        }
+        if not args.skip_hidx:
+            h, rounds, st3 = oracle.hidx(g, args.threads)
+            out.update({"hidx_sha256": digest(h), "hidx_kmax": int(h.max()) if g.m else 0,
+                        "hidx_rounds_jacobi": rounds, "hidx_stats": st3})
+        path = os.path.join(HERE, f"{wl}.json")
+        with open(path, "w") as f:
+            json.dump(out, f, indent=1, sort_keys=True)
+        print("wrote", path)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,169 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle:
+bit-exact uint64 totals, supports and bitruss numbers."""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import gpu_bitruss_amd as gb
+from oracle import oracle
+from tests import graphs as G
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def say(msg: str) -> None:
+    print(f"[gpu-parity] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def digest(a) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<u8").tobytes()).hexdigest()
+
+
+def check_all(dev, g, name, threads=8, algos=("bfc", "ebfc", "msp", "hidx")):
+    say(f"{name}: m={g.m} n={g.n}")
+    dg = dev.upload(g)
+    try:
+        osup, ototal, osmax, _ = oracle.ebfc(g, threads)
+        if "bfc" in algos:
+            assert dg.bfc() == ototal, f"{name}: bfc total"
+        if "ebfc" in algos:
+            sup, total, smax = dg.ebfc()
+            assert total == ototal and smax == osmax, f"{name}: ebfc totals"
+            assert np.array_equal(sup, osup), f"{name}: ebfc supports"
+        if "msp" in algos:
+            res, levels = dg.msp()
+            ores, olevels, _ = oracle.msp(g, threads)
+            assert np.array_equal(res, ores), f"{name}: msp results"
+            assert levels == olevels, f"{name}: msp levels"
+        if "hidx" in algos:
+            h, _ = dg.hidx()
+            oh, _, _ = oracle.hidx(g, threads)
+            assert np.array_equal(h, oh), f"{name}: hidx results"
+    finally:
+        dg.free()
+
+
+def test_aw_known_answers(dev):
+    g = G.from_pairs(G.AW_EDGES)
+    dg = dev.upload(g)
+    assert dg.bfc() == 4
+    sup, total, smax = dg.ebfc()
+    assert sup.tolist() == G.AW_SUPPORT and total == 4 and smax == 3
+    res, levels = dg.msp()
+    assert res.tolist() == G.AW_BITRUSS and levels == 2
+    h, _ = dg.hidx()
+    assert h.tolist() == G.AW_BITRUSS
+    dg.free()
+
+
+def test_quirk7(dev):
+    g = G.from_pairs(G.QUIRK7_EDGES)
+    dg = dev.upload(g)
+    res, levels = dg.msp()
+    assert res.tolist() == G.QUIRK7_MSP and levels == 2
+    h, _ = dg.hidx()
+    assert h.tolist() == G.QUIRK7_HIDX
+    dg.free()
+
+
+@pytest.mark.parametrize("a,b", [(2, 2), (3, 7), (9, 9), (40, 30)])
+def test_complete_bipartite(dev, a, b):
+    g = G.complete(a, b)
+    k = (a - 1) * (b - 1)
+    dg = dev.upload(g)
+    assert dg.bfc() == (a * (a - 1) // 2) * (b * (b - 1) // 2)
+    sup, _, _ = dg.ebfc()
+    assert set(sup.tolist()) == {k}
+    res, levels = dg.msp()
+    assert set(res.tolist()) == {k} and levels == k
+    h, _ = dg.hidx()
+    assert set(h.tolist()) == {k}
+    dg.free()
+
+
+def test_empty_and_butterfly_free(dev):
+    for pairs in ([(0, j) for j in range(50)], [(i, i) for i in range(30)] + [(i, i + 1) for i in range(29)]):
+        g = G.from_pairs(pairs)
+        dg = dev.upload(g)
+        assert dg.bfc() == 0
+        sup, total, smax = dg.ebfc()
+        assert total == 0 and smax == 0 and not sup.any()
+        res, levels = dg.msp()
+        assert not res.any() and levels == 0
+        assert not dg.hidx()[0].any()
+        dg.free()
+    g = G.from_pairs(np.zeros((0, 2), dtype=np.uint32))
+    dg = dev.upload(g)
+    assert dg.bfc() == 0
+    assert dg.msp()[1] == 0
+    dg.free()
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_small(dev, seed):
+    rng = np.random.default_rng(300 + seed)
+    g = G.random_bipartite(80 + seed, int(rng.integers(4, 40)), int(rng.integers(4, 40)),
+                           float(rng.uniform(0.2, 0.8)))
+    check_all(dev, g, f"random{seed}")
+
+
+@pytest.mark.parametrize("seed", G.QUIRK_SEEDS + G.ORDER_SEEDS)
+def test_quirk_graphs(dev, seed):
+    check_all(dev, G.quirk_graph(seed), f"quirk{seed}")
+
+
+def test_hub_graph(dev):
+    # hub degree above the LDS staging limit and hub roots in the dense path
+    check_all(dev, G.hub_graph(5, 6000, 20000, 400, 7000), "hub6000")
+
+
+@pytest.mark.parametrize("index", [0, 1, 2, 3, 17, 1234])
+def test_mix_graphs(dev, index):
+    g = gb.Graph.generate("mix_graph", index)
+    check_all(dev, g, f"mix{index}", algos=("bfc", "ebfc", "msp") if g.m > 200000 else
+              ("bfc", "ebfc", "msp", "hidx"))
+
+
+def test_small_uniform_golden(dev):
+    gold = json.load(open(os.path.join(GOLDEN, "small_uniform.json")))
+    g = gb.Graph.generate("small_uniform")
+    assert g.m == gold["m"]
+    dg = dev.upload(g)
+    sup, total, smax = dg.ebfc()
+    assert total == gold["butterflies"] and smax == gold["support_max"]
+    assert digest(sup) == gold["ebfc_sha256"]
+    res, levels = dg.msp()
+    assert digest(res) == gold["msp_sha256"] and levels == gold["msp_levels"]
+    h, _ = dg.hidx()
+    assert digest(h) == gold["hidx_sha256"]
+    dg.free()
+
+
+def test_powerlaw_2m_golden(dev):
+    path = os.path.join(GOLDEN, "powerlaw_2m.json")
+    gold = json.load(open(path))
+    g = gb.Graph.generate("powerlaw_2m")
+    assert g.m == gold["m"]
+    assert hashlib.sha256(g.neighbors.tobytes()).hexdigest() == gold["neighbors_sha256"]
+    dg = dev.upload(g)
+    say("powerlaw_2m ebfc")
+    sup, total, smax = dg.ebfc()
+    assert total == gold["butterflies"] and smax == gold["support_max"]
+    assert digest(sup) == gold["ebfc_sha256"]
+    say("powerlaw_2m msp")
+    res, levels = dg.msp()
+    assert digest(res) == gold["msp_sha256"] and levels == gold["msp_levels"]
+    if "hidx_sha256" in gold:
+        say("powerlaw_2m hidx")
+        h, _ = dg.hidx()
+        assert digest(h) == gold["hidx_sha256"]
+    dg.free()
