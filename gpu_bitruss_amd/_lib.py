@@ -69,6 +69,10 @@ class BitrussError(RuntimeError):
         self.status = status
 
 
+OPT_MSP_PATH = 1
+OPT_INDEX_MEM_PCT = 2
+MSP_PATHS = {"auto": 0, "merge": 1, "index": 2}
+
 _lib = None
 
 
@@ -105,6 +109,7 @@ def lib() -> C.CDLL:
         "bt_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
         "bt_ctx_destroy": (C.c_int, [vp]),
         "bt_ctx_last_timings": (C.c_int, [vp, C.POINTER(BtTimings)]),
+        "bt_ctx_set_option": (C.c_int, [vp, C.c_int, C.c_int64]),
         "bt_graph_upload": (C.c_int, [vp, C.POINTER(BtHostGraph), C.POINTER(vp)]),
         "bt_graph_free": (C.c_int, [vp]),
         "bt_bfc": (C.c_int, [vp, vp, u64p]),

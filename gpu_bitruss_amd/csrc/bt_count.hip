@@ -48,13 +48,16 @@ __global__ void k_rank_keys(uint32_t n, const uint32_t* __restrict__ deg, uint64
     if (x < n) keys[x] = (uint64_t(deg[x]) << 32) | x;
 }
 
-__global__ void k_rank_scatter(uint32_t n, const uint64_t* __restrict__ sorted, uint32_t* __restrict__ rank,
-                               uint32_t* __restrict__ r_deg) {
+__global__ void k_rank_scatter(uint32_t n, uint32_t u_num, const uint64_t* __restrict__ sorted,
+                               uint32_t* __restrict__ rank, uint32_t* __restrict__ r_deg,
+                               uint8_t* __restrict__ r_up) {
     uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         uint64_t const k = sorted[i];
-        rank[uint32_t(k)] = i;
+        uint32_t const x = uint32_t(k);
+        rank[x] = i;
         r_deg[i] = uint32_t(k >> 32);
+        r_up[i] = x < u_num ? 1 : 0;
     }
 }
 
@@ -192,14 +195,36 @@ struct RowView {
     const uint32_t* __restrict__ r_p;
 };
 
-// Expands the wedges of entries [i0, i1) of root row (all rooted at r) in
-// chunks of BLOCK entries. PASS 1 counts far ends into the table and returns
-// the sum of the old counts (the butterflies); PASS 2 adds cnt-1 to the
-// supports of v-w (one atomic per wedge) and u-v (aggregated per entry).
+// Optional output of the wedges that belong to blooms (far end w with
+// cnt >= 2): one record per wedge, key = (r << nb) | w, value =
+// (id(r-v) << 32) | id(v-w). Sorting the records by key groups the blooms.
+struct Emit {
+    uint64_t* keys = nullptr;
+    uint64_t* vals = nullptr;
+    unsigned long long* count = nullptr;
+    int nb = 0;
+};
+
+// Per-block scratch of expand_entries.
+template <int BLOCK>
+struct ExpandShared {
+    uint32_t start[BLOCK];
+    uint32_t pos[BLOCK];
+    uint64_t acc[BLOCK];
+    uint32_t part[BLOCK / kWave + 1];
+    uint64_t red[BLOCK / kWave + 1];
+    unsigned long long base;
+};
+
+// Expands the wedges of entries [i0, i1) of root row r in chunks of BLOCK
+// entries. PASS 1 counts far ends into the table and returns the sum of the
+// old counts (the butterflies); PASS 2 adds cnt-1 to the supports of v-w
+// (one atomic per wedge) and u-v (aggregated per entry), and emits the
+// bloom wedge records when em.keys is set.
 template <int BLOCK, int PASS, class Table>
-__device__ __forceinline__ uint64_t expand_entries(const RowView& g, uint32_t i0, uint32_t i1, const Table& tab,
-                                                   uint64_t* __restrict__ sup, uint32_t* s_start, uint32_t* s_pos,
-                                                   uint32_t* s_part, uint64_t* s_acc) {
+__device__ __forceinline__ uint64_t expand_entries(const RowView& g, uint32_t r, uint32_t i0, uint32_t i1,
+                                                   const Table& tab, uint64_t* __restrict__ sup, const Emit& em,
+                                                   ExpandShared<BLOCK>& sh) {
     uint64_t tot = 0;
     int const tid = threadIdx.x;
     for (uint32_t base = i0; base < i1; base += BLOCK) {
@@ -208,14 +233,15 @@ __device__ __forceinline__ uint64_t expand_entries(const RowView& g, uint32_t i0
         uint32_t const p = valid ? g.r_p[idx] : 0;
         uint32_t const v = valid ? g.r_nbr[idx] : 0;
         uint32_t chunk = 0;
-        uint32_t const excl = block_excl_scan<BLOCK>(p, s_part, chunk);
-        s_start[tid] = excl;
-        s_pos[tid] = valid ? g.r_off[v] : 0;
-        if (PASS == 2) s_acc[tid] = 0;
+        uint32_t const excl = block_excl_scan<BLOCK>(p, sh.part, chunk);
+        sh.start[tid] = excl;
+        sh.pos[tid] = valid ? g.r_off[v] : 0;
+        if (PASS == 2) sh.acc[tid] = 0;
         __syncthreads();
+        uint32_t my_emit = 0;
         for (uint32_t t = tid; t < chunk; t += BLOCK) {
-            int const j = upper_index(s_start, BLOCK, t);
-            uint32_t const pos = s_pos[j] + (t - s_start[j]);
+            int const j = upper_index(sh.start, BLOCK, t);
+            uint32_t const pos = sh.pos[j] + (t - sh.start[j]);
             uint32_t const w = g.r_nbr[pos];
             if (PASS == 1) {
                 tot += tab.inc(w);
@@ -223,13 +249,31 @@ __device__ __forceinline__ uint64_t expand_entries(const RowView& g, uint32_t i0
                 uint32_t const c = tab.get(w);
                 if (c > 1) {
                     atomic_add_u64(&sup[g.r_eid[pos]], uint64_t(c - 1));
-                    atomicAdd(reinterpret_cast<unsigned long long*>(&s_acc[j]), (unsigned long long)(c - 1));
+                    atomicAdd(reinterpret_cast<unsigned long long*>(&sh.acc[j]), (unsigned long long)(c - 1));
+                    my_emit++;
                 }
             }
         }
         __syncthreads();
         if (PASS == 2) {
-            if (valid && s_acc[tid]) atomic_add_u64(&sup[g.r_eid[idx]], s_acc[tid]);
+            if (valid && sh.acc[tid]) atomic_add_u64(&sup[g.r_eid[idx]], sh.acc[tid]);
+            if (em.keys) {
+                uint32_t total_emit = 0;
+                uint32_t const ex = block_excl_scan<BLOCK>(my_emit, sh.part, total_emit);
+                if (tid == 0) sh.base = total_emit ? atomicAdd(em.count, (unsigned long long)total_emit) : 0ull;
+                __syncthreads();
+                uint64_t at = sh.base + ex;
+                for (uint32_t t = tid; t < chunk; t += BLOCK) {
+                    int const j = upper_index(sh.start, BLOCK, t);
+                    uint32_t const pos = sh.pos[j] + (t - sh.start[j]);
+                    uint32_t const w = g.r_nbr[pos];
+                    if (tab.get(w) > 1) {
+                        em.keys[at] = (uint64_t(r) << em.nb) | w;
+                        em.vals[at] = (uint64_t(g.r_eid[base + j]) << 32) | g.r_eid[pos];
+                        at++;
+                    }
+                }
+            }
             __syncthreads();
         }
     }
@@ -241,13 +285,9 @@ template <int BLOCK, uint32_t CAP>
 __global__ __launch_bounds__(BLOCK) void k_count_lds(const uint32_t* __restrict__ roots, uint32_t nroots,
                                                      RowView g, const uint32_t* __restrict__ r_lo,
                                                      uint64_t* __restrict__ sup, uint64_t* __restrict__ total,
-                                                     int cb, int per_edge) {
+                                                     int cb, int per_edge, Emit em) {
     __shared__ uint32_t tab[CAP];
-    __shared__ uint32_t s_start[BLOCK];
-    __shared__ uint32_t s_pos[BLOCK];
-    __shared__ uint64_t s_acc[BLOCK];
-    __shared__ uint32_t s_part[BLOCK / kWave + 1];
-    __shared__ uint64_t s_red[BLOCK / kWave + 1];
+    __shared__ ExpandShared<BLOCK> sh;
     constexpr int kLog = __builtin_ctz(CAP);
     LdsHash h{tab, CAP - 1, 32 - kLog, cb};
     uint64_t tot = 0;
@@ -256,11 +296,11 @@ __global__ __launch_bounds__(BLOCK) void k_count_lds(const uint32_t* __restrict_
         uint32_t const b = g.r_off[r], lo = r_lo[r];
         for (uint32_t i = threadIdx.x; i < CAP; i += BLOCK) tab[i] = 0;
         __syncthreads();
-        tot += expand_entries<BLOCK, 1>(g, b, b + lo, h, sup, s_start, s_pos, s_part, s_acc);
-        if (per_edge) expand_entries<BLOCK, 2>(g, b, b + lo, h, sup, s_start, s_pos, s_part, s_acc);
+        tot += expand_entries<BLOCK, 1>(g, r, b, b + lo, h, sup, em, sh);
+        if (per_edge) expand_entries<BLOCK, 2>(g, r, b, b + lo, h, sup, em, sh);
         __syncthreads();
     }
-    uint64_t const s = block_sum_u64<BLOCK>(tot, s_red);
+    uint64_t const s = block_sum_u64<BLOCK>(tot, sh.red);
     if (threadIdx.x == 0 && s) atomic_add_u64(total, s);
 }
 
@@ -274,20 +314,17 @@ struct HubItem {
 template <int BLOCK, int PASS>
 __global__ __launch_bounds__(BLOCK) void k_count_dense(const HubItem* __restrict__ items, uint32_t nitems,
                                                        RowView g, uint32_t* __restrict__ pool,
-                                                       uint64_t* __restrict__ sup, uint64_t* __restrict__ total) {
-    __shared__ uint32_t s_start[BLOCK];
-    __shared__ uint32_t s_pos[BLOCK];
-    __shared__ uint64_t s_acc[BLOCK];
-    __shared__ uint32_t s_part[BLOCK / kWave + 1];
-    __shared__ uint64_t s_red[BLOCK / kWave + 1];
+                                                       uint64_t* __restrict__ sup, uint64_t* __restrict__ total,
+                                                       Emit em) {
+    __shared__ ExpandShared<BLOCK> sh;
     uint64_t tot = 0;
     for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
         HubItem const h = items[it];
         DenseTable t{pool + h.toff};
-        tot += expand_entries<BLOCK, PASS>(g, h.i0, h.i1, t, sup, s_start, s_pos, s_part, s_acc);
+        tot += expand_entries<BLOCK, PASS>(g, h.r, h.i0, h.i1, t, sup, em, sh);
     }
     if (PASS == 1) {
-        uint64_t const s = block_sum_u64<BLOCK>(tot, s_red);
+        uint64_t const s = block_sum_u64<BLOCK>(tot, sh.red);
         if (threadIdx.x == 0 && s) atomic_add_u64(total, s);
     }
 }
@@ -326,7 +363,7 @@ int bt_prep_priority(bt_dgraph* dg) {
     // rank = position in (degree, id) order
     k_rank_keys<<<grid_for(n, T, 1u << 30), T, 0, s>>>(n, dg->deg, dg->r_key);
     BT_HIP_TRY(rocprim::radix_sort_keys(c->sort_tmp.ptr, t1, dg->r_key, dg->r_key2, n, 0, 64, s));
-    k_rank_scatter<<<grid_for(n, T, 1u << 30), T, 0, s>>>(n, dg->r_key2, dg->rank, dg->list0);
+    k_rank_scatter<<<grid_for(n, T, 1u << 30), T, 0, s>>>(n, dg->u_num, dg->r_key2, dg->rank, dg->list0, dg->r_up);
     BT_HIP_TRY(rocprim::exclusive_scan(c->sort_tmp.ptr, t3, dg->list0, dg->r_off, 0u, n,
                                        rocprim::plus<uint32_t>(), s));
     k_set_last<<<1, 64, 0, s>>>(dg->r_off, n, 2 * m);
@@ -343,7 +380,7 @@ int bt_prep_priority(bt_dgraph* dg) {
     return BT_OK;
 }
 
-int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total_out, uint64_t* smax_out) {
+int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total_out, uint64_t* smax_out, bool emit) {
     bt_ctx* c = dg->ctx;
     hipStream_t s = c->stream;
     uint32_t const n = dg->n, m = dg->m;
@@ -380,15 +417,39 @@ int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total_out, uint64_t* s
     uint64_t const na = cnts[0], nb = cnts[1], nc = cnts[2];
     c->last.wedges_count = cnts[3];
     RowView g{dg->r_off, dg->r_nbr, dg->r_eid, dg->r_p};
+    Emit em;
+    c->ix.R = kNoIndex;
+    if (emit && per_edge) {
+        // at most one record per priority wedge; the index (records, sorted
+        // copy, bloom ids, incidences) takes about 44 bytes per wedge
+        bt_index& ix = c->ix;
+        size_t const cap = std::max<uint64_t>(cnts[3], 1);
+        size_t free_b = 0, total_b = 0;
+        BT_HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+        size_t const have = ix.keys0.bytes + ix.keys1.bytes + ix.vals0.bytes + ix.vals1.bytes + ix.bid.bytes +
+                            ix.inc.bytes;
+        if (double(cap) * 44.0 > (double(free_b) + double(have)) * c->index_mem_frac || cap >= (1ull << 31))
+            emit = false;
+    }
+    if (emit && per_edge) {
+        bt_index& ix = c->ix;
+        size_t const cap = std::max<uint64_t>(cnts[3], 1);
+        BT_TRY(bt_buf_reserve(ix.keys0, cap * sizeof(uint64_t)));
+        BT_TRY(bt_buf_reserve(ix.vals0, cap * sizeof(uint64_t)));
+        em.keys = static_cast<uint64_t*>(ix.keys0.ptr);
+        em.vals = static_cast<uint64_t*>(ix.vals0.ptr);
+        em.count = reinterpret_cast<unsigned long long*>(dg->ctl + 10);
+        em.nb = bits_for(n);
+    }
     uint64_t* total = dg->ctl + 8;
     unsigned const max_grid = unsigned(c->cu_count) * 16;
     BT_TRY(tm.start(2));
     if (na)
         k_count_lds<kWave, kCapA><<<grid_for(na, 1, max_grid * 4), kWave, 0, s>>>(la, uint32_t(na), g, dg->r_lo,
-                                                                               dg->sup, total, cb, per_edge);
+                                                                               dg->sup, total, cb, per_edge, em);
     if (nb)
         k_count_lds<kBlockB, kCapB><<<grid_for(nb, 1, max_grid), kBlockB, 0, s>>>(lb, uint32_t(nb), g, dg->r_lo,
-                                                                               dg->sup, total, cb, per_edge);
+                                                                               dg->sup, total, cb, per_edge, em);
     BT_HIP_TRY(hipGetLastError());
     if (nc) {
         // hub roots: build (root, entry chunk) items on the host, batch them
@@ -426,10 +487,11 @@ int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total_out, uint64_t* s
             BT_HIP_TRY(hipMemsetAsync(pool, 0, used * sizeof(uint32_t), s));
             BT_HIP_TRY(hipMemcpyAsync(ditems, items.data(), items.size() * sizeof(HubItem), hipMemcpyHostToDevice, s));
             unsigned const gC = grid_for(items.size(), 1, max_grid);
-            k_count_dense<kBlockC, 1><<<gC, kBlockC, 0, s>>>(ditems, uint32_t(items.size()), g, pool, dg->sup, total);
+            k_count_dense<kBlockC, 1><<<gC, kBlockC, 0, s>>>(ditems, uint32_t(items.size()), g, pool, dg->sup, total,
+                                                             em);
             if (per_edge)
                 k_count_dense<kBlockC, 2><<<gC, kBlockC, 0, s>>>(ditems, uint32_t(items.size()), g, pool, dg->sup,
-                                                                 total);
+                                                                 total, em);
             BT_HIP_TRY(hipGetLastError());
             // the host vector `items` is reused only after this batch's copy
             BT_HIP_TRY(hipStreamSynchronize(s));
@@ -443,10 +505,11 @@ int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total_out, uint64_t* s
         BT_HIP_TRY(hipGetLastError());
     }
     BT_TRY(tm.start(4));
-    uint64_t res[2];
-    BT_TRY(bt_read_ctl(dg, 8, 2, res));
+    uint64_t res[3];
+    BT_TRY(bt_read_ctl(dg, 8, 3, res));
     if (total_out) *total_out = res[0];
     if (smax_out) *smax_out = res[1];
+    c->ix.R = em.keys ? res[2] : kNoIndex;
     dg->has_support = per_edge;
     c->last.prep_ms = tm.ms(0, 1);
     c->last.count_kernel_ms = tm.ms(2, 3);

@@ -322,11 +322,24 @@ __global__ __launch_bounds__(kPeelBlock) void k_hidx_round(uint32_t m, Csr g, co
 
 }  // namespace
 
+// msp entry: supports (and, for the index path, bloom records) first, then
+// the peel. The index path needs cooperative launch and room for the index
+// (about 44 bytes per priority wedge); otherwise the merge path runs.
 int bt_msp_impl(bt_dgraph* dg, uint32_t* levels_out) {
+    bt_ctx* c = dg->ctx;
+    bool const want_index = c->coop && c->msp_path != BT_MSP_PATH_MERGE;
+    BT_TRY(bt_count_impl(dg, true, nullptr, nullptr, want_index));
+    if (want_index && c->ix.R != kNoIndex) return bt_msp_index_peel(dg, levels_out);
+    if (c->msp_path == BT_MSP_PATH_INDEX) return BT_ERR_BUDGET;
+    return bt_msp_merge_peel(dg, levels_out);
+}
+
+// Merge-path peel: one workgroup per frontier edge per sub-round, the host
+// steps the levels and sub-rounds (process_sub_level, bitruss_msp.cu:290-309).
+int bt_msp_merge_peel(bt_dgraph* dg, uint32_t* levels_out) {
     bt_ctx* c = dg->ctx;
     hipStream_t s = c->stream;
     uint32_t const m = dg->m;
-    BT_TRY(bt_count_impl(dg, true, nullptr, nullptr));
     bt_timings const count_t = c->last;
     bt_timer tm(c);
     BT_TRY(tm.start(5));

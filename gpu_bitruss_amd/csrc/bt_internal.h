@@ -35,6 +35,24 @@ struct bt_buf {
 int bt_buf_reserve(bt_buf& b, size_t bytes);
 void bt_buf_free(bt_buf& b);
 
+// Bloom index of a graph (wedges grouped by (root, far end) with >= 2
+// middles), rebuilt by each msp call; buffers are reused across calls.
+struct bt_index {
+    bt_buf keys0, keys1;  // record keys (emission order, sorted)
+    bt_buf vals0, vals1;  // record values; vals1 = sorted middles (id(r-v) << 32 | id(v-w))
+    bt_buf bid;           // u32 per record: inclusive head count (bloom id + 1)
+    bt_buf bstart;        // u32 per bloom: first record, plus a sentinel R
+    bt_buf bk;            // u32 per record, used at bloom starts: live middles | root-is-upper << 31
+    bt_buf bclaim;        // u32 per record, used at bloom starts: last claiming sub-round
+    bt_buf inc_off;       // u32 per edge + 1: incidence offsets
+    bt_buf inc_cur;       // u32 per edge: fill cursors
+    bt_buf inc;           // u32 per incidence: first record of the bloom
+    bt_buf touched;       // u32 3 x blooms: claimed-bloom lists
+    bt_buf live;          // u32 per edge: per-block live-edge segments
+    bt_buf seg;           // u32 per block + 1: segment offsets, then counts
+    uint64_t R = 0, NB = 0;
+};
+
 struct bt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -47,6 +65,10 @@ struct bt_ctx {
     bt_buf pool;       // dense per-root count tables for hub roots
     bt_buf pinned_dummy;
     uint64_t* h_pinned = nullptr;  // small pinned host buffer for read-backs
+    bt_index ix;
+    int coop = 0;                  // cooperative launch supported
+    int msp_path = 0;              // BT_MSP_PATH_*
+    double index_mem_frac = 0.5;   // share of free device memory the index may use
 };
 
 // Device-resident graph plus the per-graph scratch of the algorithms.
@@ -65,6 +87,7 @@ struct bt_dgraph {
     uint32_t* r_nbr = nullptr;  // 2m  : neighbour ranks, ascending per row
     uint32_t* r_eid = nullptr;  // 2m
     uint32_t* r_lo = nullptr;   // n   : neighbours of lower rank (a row prefix)
+    uint8_t* r_up = nullptr;    // n   : rank belongs to an upper vertex
     uint32_t* r_p = nullptr;    // 2m  : per lower neighbour v of r, #N(v) below r
     uint64_t* r_w = nullptr;    // n   : priority wedges rooted at r
     uint64_t* sup = nullptr;    // m   : ebfc supports (then msp working supports)
@@ -91,7 +114,12 @@ struct bt_timer {
 // Algorithm entry points implemented in the .hip files (inputs checked by
 // the C-ABI layer).
 int bt_prep_priority(bt_dgraph* dg);
-int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total, uint64_t* support_max);
+int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total, uint64_t* support_max, bool emit = false);
+int bt_index_build(bt_dgraph* dg);
+int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels);
+int bt_msp_merge_peel(bt_dgraph* dg, uint32_t* levels);
+// ix.R value meaning "records were not emitted" (index path unavailable)
+constexpr uint64_t kNoIndex = ~0ull;
 int bt_msp_impl(bt_dgraph* dg, uint32_t* levels);
 int bt_hidx_impl(bt_dgraph* dg, uint32_t* rounds);
 

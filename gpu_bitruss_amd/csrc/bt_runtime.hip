@@ -128,6 +128,7 @@ int bt_ctx_create(int device, bt_ctx** out) {
         return BT_ERR_DEVICE;
     }
     c->cu_count = prop.multiProcessorCount;
+    c->coop = prop.cooperativeLaunch;
     c->lds_per_block = prop.sharedMemPerBlock;
     for (auto& e : c->ev) {
         if (hipEventCreate(&e) != hipSuccess) {
@@ -150,12 +151,32 @@ int bt_ctx_destroy(bt_ctx* c) {
     bt_buf_free(c->sort_tmp);
     bt_buf_free(c->pool);
     bt_buf_free(c->pinned_dummy);
+    bt_buf* ixb[] = {&c->ix.keys0, &c->ix.keys1, &c->ix.vals0, &c->ix.vals1, &c->ix.bid, &c->ix.bstart,
+                     &c->ix.bk, &c->ix.bclaim, &c->ix.inc_off, &c->ix.inc_cur, &c->ix.inc, &c->ix.touched,
+                     &c->ix.live, &c->ix.seg};
+    for (bt_buf* b : ixb) bt_buf_free(*b);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return BT_OK;
+}
+
+int bt_ctx_set_option(bt_ctx* ctx, int option, int64_t value) {
+    if (!ctx) return BT_ERR_INVALID;
+    switch (option) {
+        case BT_OPT_MSP_PATH:
+            if (value < BT_MSP_PATH_AUTO || value > BT_MSP_PATH_INDEX) return BT_ERR_INVALID;
+            ctx->msp_path = int(value);
+            return BT_OK;
+        case BT_OPT_INDEX_MEM_PCT:
+            if (value < 1 || value > 95) return BT_ERR_INVALID;
+            ctx->index_mem_frac = double(value) / 100.0;
+            return BT_OK;
+        default:
+            return BT_ERR_INVALID;
+    }
 }
 
 int bt_ctx_last_timings(const bt_ctx* ctx, bt_timings* out) {
@@ -171,7 +192,7 @@ int bt_graph_free(bt_dgraph* dg) {
         (void)hipStreamSynchronize(dg->ctx->stream);
     }
     void* ptrs[] = {dg->nbr,  dg->off,   dg->deg,   dg->edges, dg->eids,  dg->rank,
-                    dg->r_off, dg->r_nbr, dg->r_eid, dg->r_lo,  dg->r_p,   dg->r_w,
+                    dg->r_off, dg->r_nbr, dg->r_eid, dg->r_lo,  dg->r_p,   dg->r_w, dg->r_up,
                     dg->sup,  dg->res,   dg->aux,   dg->state, dg->list0, dg->list1,
                     dg->ctl,  dg->r_key, dg->r_key2, dg->r_val2};
     for (void* p : ptrs)
@@ -209,6 +230,7 @@ int bt_graph_upload(bt_ctx* ctx, const bt_host_graph* g, bt_dgraph** out) {
     A(dalloc(&dg->r_nbr, 2 * m));
     A(dalloc(&dg->r_eid, 2 * m));
     A(dalloc(&dg->r_lo, n));
+    A(dalloc(&dg->r_up, n));
     A(dalloc(&dg->r_p, 2 * m));
     A(dalloc(&dg->r_w, n + 1));
     A(dalloc(&dg->sup, m));

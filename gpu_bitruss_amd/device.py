@@ -32,6 +32,14 @@ class Device:
             raise RuntimeError("device context is closed")
         return self._ctx
 
+    def set_msp_path(self, path: str) -> None:
+        """"auto" (bloom-index peel when it fits), "merge" or "index"."""
+        _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(
+            self.handle, _lib.OPT_MSP_PATH, _lib.MSP_PATHS[path]))
+
+    def set_index_mem_pct(self, pct: int) -> None:
+        _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(self.handle, _lib.OPT_INDEX_MEM_PCT, pct))
+
     def upload(self, g: Graph) -> "DeviceGraph":
         return DeviceGraph(self, g)
 

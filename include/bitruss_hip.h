@@ -74,6 +74,18 @@ int bt_ctx_create(int device, bt_ctx** out);
 int bt_ctx_destroy(bt_ctx* ctx);
 int bt_ctx_last_timings(const bt_ctx* ctx, bt_timings* out);
 
+/* Context options (no counterpart in the reference, whose launch shapes are
+ * compile-time macros in src/util/config.h:6-13).
+ *   BT_OPT_MSP_PATH        BT_MSP_PATH_AUTO (default): bloom-index peel in one
+ *                          persistent launch when the index fits, else the
+ *                          merge peel; _MERGE / _INDEX force one path
+ *                          (_INDEX fails with BT_ERR_BUDGET when it cannot run)
+ *   BT_OPT_INDEX_MEM_PCT   share of free device memory the index may use
+ *                          (1..95, default 50) */
+enum { BT_OPT_MSP_PATH = 1, BT_OPT_INDEX_MEM_PCT = 2 };
+enum { BT_MSP_PATH_AUTO = 0, BT_MSP_PATH_MERGE = 1, BT_MSP_PATH_INDEX = 2 };
+int bt_ctx_set_option(bt_ctx* ctx, int option, int64_t value);
+
 /* Copy a host CSR (the reference Graph arrays, graph.h:43-51) to the
  * device and keep it resident. Replaces the per-call cudaMalloc/cudaMemcpy
  * of bfc_vpp.cu:233-262, bitruss_hidx.cu:196-211, bitruss_msp.cu:447-465. */

@@ -133,6 +133,19 @@ def test_mix_graphs(dev, index):
               ("bfc", "ebfc", "msp", "hidx"))
 
 
+@pytest.mark.parametrize("path", ["merge", "index"])
+def test_msp_paths_agree(dev, path):
+    """Both peel paths against the oracle on graphs with many sub-rounds."""
+    try:
+        dev.set_msp_path(path)
+        for seed in G.QUIRK_SEEDS[:3] + G.ORDER_SEEDS[:2]:
+            check_all(dev, G.quirk_graph(seed), f"{path}-quirk{seed}", algos=("msp",))
+        check_all(dev, gb.Graph.generate("mix_graph", 2), f"{path}-mix2", algos=("msp",))
+        check_all(dev, G.hub_graph(5, 6000, 20000, 400, 7000), f"{path}-hub", algos=("msp",))
+    finally:
+        dev.set_msp_path("auto")
+
+
 def test_small_uniform_golden(dev):
     gold = json.load(open(os.path.join(GOLDEN, "small_uniform.json")))
     g = gb.Graph.generate("small_uniform")
