@@ -14,7 +14,8 @@
 //   bstart[NB+1]     first record of each bloom (sentinel R)
 //   bk[R]            at a bloom's first record s: number of middles |
 //                    (r is an upper vertex) << 31; a bloom is named by s
-//   bclaim[R]        at s: claim stamp of the peel (0 here)
+//   bclaim[R + ...]  at s: claim stamp of the peel (0 here); then the
+//                    peel's dummy stamps (kClaimDummiesPerBlock per block)
 //   inc_off[m+1]     per-edge offsets into inc
 //   inc[2R]          first records of the blooms holding the edge
 #include <hip/hip_runtime.h>
@@ -101,7 +102,7 @@ int bt_index_build(bt_dgraph* dg) {
         BT_HIP_TRY(hipMemsetAsync(inc_off, 0, sizeof(uint32_t) * (size_t(m) + 1), s));
         BT_TRY(bt_buf_reserve(ix.bstart, 64));
         BT_TRY(bt_buf_reserve(ix.bk, 64));
-        BT_TRY(bt_buf_reserve(ix.bclaim, 64));
+        BT_TRY(bt_buf_reserve(ix.bclaim, sizeof(uint32_t) * size_t(c->cu_count) * kClaimDummiesPerBlock));
         BT_TRY(bt_buf_reserve(ix.touched, 64));
         BT_TRY(bt_buf_reserve(ix.inc, 64));
         BT_TRY(bt_buf_reserve(ix.vals1, 64));
@@ -131,7 +132,7 @@ int bt_index_build(bt_dgraph* dg) {
     ix.NB = nbloom;
     BT_TRY(bt_buf_reserve(ix.bstart, sizeof(uint32_t) * (size_t(nbloom) + 1)));
     BT_TRY(bt_buf_reserve(ix.bk, sizeof(uint32_t) * R));
-    BT_TRY(bt_buf_reserve(ix.bclaim, sizeof(uint32_t) * R));
+    BT_TRY(bt_buf_reserve(ix.bclaim, sizeof(uint32_t) * (R + size_t(c->cu_count) * kClaimDummiesPerBlock)));
     BT_TRY(bt_buf_reserve(ix.touched, sizeof(uint32_t) * 3 * std::max<size_t>(nbloom, 1)));
     BT_TRY(bt_buf_reserve(ix.inc, sizeof(uint32_t) * 2 * R));
     auto* bstart = static_cast<uint32_t*>(ix.bstart.ptr);

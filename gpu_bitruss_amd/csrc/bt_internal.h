@@ -35,6 +35,10 @@ struct bt_buf {
 int bt_buf_reserve(bt_buf& b, size_t bytes);
 void bt_buf_free(bt_buf& b);
 
+// Claim stamps past the R real ones that the peel's claim steps use for
+// lanes with nothing to claim (one per thread and unroll slot of a block).
+constexpr uint32_t kClaimDummiesPerBlock = 2048;
+
 // Bloom index of a graph (wedges grouped by (root, far end) with >= 2
 // middles), rebuilt by each msp call; buffers are reused across calls.
 struct bt_index {
@@ -47,7 +51,8 @@ struct bt_index {
     bt_buf inc_off;       // u32 per edge + 1: incidence offsets
     bt_buf inc_cur;       // u32 per edge: fill cursors
     bt_buf inc;           // u32 per incidence: first record of the bloom
-    bt_buf touched;       // u32 3 x blooms: claimed-bloom lists
+    bt_buf touched;       // u32 3 x blooms: shared claimed-bloom lists
+    bt_buf area;          // u32 3 x blocks x cap: per-block claimed-bloom lists
     bt_buf live;          // u32 per edge: per-block live-edge segments
     bt_buf seg;           // u32 per block + 1: segment offsets, then counts
     uint64_t R = 0, NB = 0;

@@ -28,15 +28,21 @@
 // counter then advances by two so those edges count as done).
 //
 // Control: one cooperative launch, one block per CU; phases are separated
-// by a grid barrier and every block takes the same decision after each
-// barrier from the same control words. Per sub-round one barrier; per level
-// one more (collect). No word is written by every block in a phase except
-// the barrier counter and one claim-list reservation per block:
-//   - claimed blooms are staged per block in LDS and appended with one
-//     reservation at the end of the phase;
-//   - per-block minima (kept supports, decremented supports) go to private
-//     slots; the last block to arrive at a level's final barrier reduces
-//     them into the next level.
+// by a grid barrier, and after each barrier every block reads the same
+// per-block words and takes the same decision. Nothing in a phase is
+// written by many blocks to one word:
+//   - the barrier: each block stores its arrival epoch in its own slot,
+//     block 0 watches all slots (one thread each) and releases with one
+//     store (about 1.6 us for 256 blocks; a shared counter serialises the
+//     256 arrivals at about 12 ns each);
+//   - claimed blooms: each block appends to its own list and publishes its
+//     length; readers locate entry t by a search over the block prefix
+//     (only a block that overflows its list reserves in a shared one);
+//   - the next level: per-block minima (kept supports, decremented
+//     supports) in private slots, reduced by every block at a level's end.
+// The blooms a frontier edge belongs to are claimed by the whole block at
+// the end of a phase (its incidence lists concatenated and walked by 512
+// lanes), so a hub edge's long list does not serialise one wave.
 // Coherence across the 8 XCDs without cache-wide fences: every word that
 // one block writes and another reads later is accessed with relaxed
 // device-scope atomics, and a barrier first waits for the block's own
@@ -55,22 +61,24 @@ using namespace bt;
 
 namespace {
 
-constexpr int kMB = 512;           // block size (one block per CU)
-constexpr int kWPB = kMB / kWave;  // waves per block
-constexpr int kC2Cap = 128;        // both-frontier middles kept in LDS per wave
-constexpr int kClaimUnroll = 4;    // incidences per lane per claim step
-constexpr int kXCap = 128;         // crossing edges buffered per wave
-constexpr int kStage = 4096;       // claimed blooms staged per block
+constexpr int kMB = 512;                // block size (one block per CU)
+constexpr int kWPB = kMB / kWave;       // waves per block
+constexpr int kC2Cap = 128;             // both-frontier middles kept in LDS per wave
+constexpr int kClaimUnroll = 4;         // incidences per lane per claim step
+static_assert(kMB * kClaimUnroll == int(kClaimDummiesPerBlock), "dummy claim stamps per block");
+constexpr int kXCap = 128;              // frontier edges buffered per wave
+constexpr int kBL = kWPB * kXCap;       // frontier edges claimed together by a block
+constexpr uint32_t kBlkCap = 16384;     // claimed blooms a block keeps in its own list
+constexpr int kSp = 32;                 // u32 spacing of the barrier slots (128 B)
 constexpr uint64_t kInf = ~0ull;
 constexpr uint64_t kBarrierLimit = 30ull * 100000000ull;  // 30 s of the 100 MHz clock
+constexpr unsigned kRelErr = 0x80000000u;                   // release word: error bit
 
 struct MspCtl {
-    unsigned bar_count;
-    unsigned bar_gen;
+    unsigned rel;                    // barrier release epoch | kRelErr
     unsigned err;
-    unsigned pad0;
-    unsigned touched_n[4];           // claim-list lengths by sub-round % 3
-    unsigned long long next_level;   // set by the last block at a level's end
+    unsigned ovf_n[4];               // shared-list claims by sub-round % 3
+    unsigned pad[2];
     unsigned long long subrounds;
     unsigned long long levels;
     unsigned long long level_max;
@@ -83,8 +91,8 @@ static_assert(sizeof(MspCtl) <= 64 * sizeof(uint64_t), "control block overlays d
 
 struct MspArgs {
     uint32_t m;
-    uint32_t nbloom;       // capacity of one claim list
     uint32_t R;            // records (bloom middles)
+    uint32_t nbloom;       // capacity of one shared claim list
     uint32_t diag;         // record per-phase diagnostics
     uint64_t* sup;
     uint64_t* res;
@@ -93,11 +101,15 @@ struct MspArgs {
     const uint32_t* inc;   // per incidence: first record of the bloom
     uint64_t* rec;         // middles, compacted in place
     uint32_t* hdr;         // at a bloom's first record: live middles | upper << 31
-    uint32_t* claim;       // at a bloom's first record: last claiming sub-round
-    uint32_t* touched;     // 3 claim lists of bloom first records
+    uint32_t* claim;       // at a bloom's first record: last claiming sub-round;
+                           // then G * kMB * kClaimUnroll dummy stamps
+    uint32_t* area;        // [3][G][kBlkCap] per-block claim lists by sub-round % 3
+    uint32_t* ovf;         // [3][nbloom] shared claim lists (block list overflow)
+    uint32_t* cnt;         // [3][G] per-block claim-list lengths
+    uint64_t* slots;       // [2 level parities][2][G]: min kept, min decremented support
+    uint32_t* arrive;      // [G * kSp] barrier arrival epochs
     uint32_t* live;
     uint32_t* live_cnt;
-    uint64_t* slots;       // per block: [0, G) min kept support, [G, 2G) min decremented support
     MspCtl* ctl;
     uint64_t* trace;       // optional: per sub-round (claimed, slowest-wave ticks)
     uint32_t trace_cap;
@@ -149,179 +161,246 @@ __device__ __forceinline__ uint64_t block_min_all(uint64_t v, uint64_t* s_red) {
     return r;
 }
 
-// Control words a block needs after a barrier; thread 0 reads them once
-// (every thread of the grid loading one word would serialise on it).
-struct Snap {
-    unsigned touched;               // claim-list length of the next peel
-    int ok;
-    unsigned long long next_level;
-};
-
-enum SyncMode { kSyncPlain = 0, kSyncLevelIfIdle = 1, kSyncLevel = 2 };
-
-// Grid barrier for a cooperative launch (every block arrives once per
-// phase). The last block to arrive reduces the per-block minima into
-// next_level when the level ends (mode kSyncLevel, or kSyncLevelIfIdle when
-// claim list `idx` is empty). Returns false (and sets err) when a wait
-// exceeds kBarrierLimit, so a broken invariant ends the kernel instead of
-// hanging it. Afterwards `snap` holds touched_n[idx] and next_level.
-__device__ __forceinline__ bool grid_sync(const MspArgs& a, unsigned nblocks, int mode, unsigned idx, Snap* snap,
-                          uint64_t* s_red) {
-    __shared__ int s_last;
-    __shared__ unsigned s_gen;
+// Grid barrier for a cooperative launch (every block calls it with the same
+// epoch sequence 1, 2, ...). Waits are bounded by kBarrierLimit: a timeout
+// sets the error word and the error bit of the release, and every block
+// returns false, so a broken invariant ends the kernel instead of hanging.
+__device__ __forceinline__ bool grid_sync(const MspArgs& a, unsigned G, unsigned epoch, int* s_ok) {
     MspCtl* ctl = a.ctl;
     drain();
     __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned const gen = ald(&ctl->bar_gen);
-        drain();
-        unsigned const arrived = afetch_add(&ctl->bar_count, 1u);
-        s_gen = gen;
-        int last = arrived == nblocks - 1;
-        if (last && mode == kSyncLevelIfIdle) last = ald(&ctl->touched_n[idx]) == 0 ? 2 : 1;
-        else if (last) last = mode == kSyncLevel ? 2 : 1;
-        s_last = last;
-    }
-    __syncthreads();
-    int const last = s_last;
-    if (last == 2) {
-        // level ends: next level = min over blocks of kept / decremented supports
-        uint64_t v = kInf;
-        for (uint32_t i = threadIdx.x; i < 2 * nblocks; i += blockDim.x) {
-            uint64_t const x = ald(&a.slots[i]);
-            v = x < v ? x : v;
-        }
-        v = block_min_all(v, s_red);
-        if (threadIdx.x == 0) ast(&ctl->next_level, (unsigned long long)v);
-    }
-    if (threadIdx.x == 0) {
+    if (blockIdx.x == 0) {
         int ok = 1;
-        if (last) {
-            ast(&ctl->bar_count, 0u);
-            drain();
-            afetch_add(&ctl->bar_gen, 1u);
-        } else {
+        uint32_t const t = threadIdx.x;
+        if (t != 0 && t < G) {
             uint64_t const t0 = wall_clock64();
-            while (ald(&ctl->bar_gen) == s_gen) {
-                __builtin_amdgcn_s_sleep(1);
+            while (ald(&a.arrive[t * kSp]) != epoch) {
                 if (wall_clock64() - t0 > kBarrierLimit) {
-                    ast(&ctl->err, 1u);
                     ok = 0;
                     break;
                 }
             }
         }
-        snap->ok = ok && ald(&ctl->err) == 0;
-        snap->touched = ald(&ctl->touched_n[idx]);
-        snap->next_level = ald(&ctl->next_level);
+        ok = __syncthreads_and(ok);
+        if (t == 0) {
+            if (!ok) ast(&ctl->err, 1u);
+            ast(&ctl->rel, epoch | (ok ? 0u : kRelErr));
+        }
+        return ok != 0;
+    }
+    if (threadIdx.x == 0) {
+        ast(&a.arrive[blockIdx.x * kSp], epoch);
+        int ok = 1;
+        uint64_t const t0 = wall_clock64();
+        unsigned r;
+        while (((r = ald(&ctl->rel)) & ~kRelErr) != epoch) {
+            __builtin_amdgcn_s_sleep(1);
+            if (wall_clock64() - t0 > kBarrierLimit) {
+                ast(&ctl->err, 1u);
+                ok = 0;
+                break;
+            }
+        }
+        *s_ok = ok && !(r & kRelErr);
     }
     __syncthreads();
-    return snap->ok != 0;
+    return *s_ok != 0;
 }
 
-// Block-local staging of claimed blooms; one reservation per phase in the
-// global claim list (overflow beyond the LDS buffer is written directly).
-struct Stage {
-    uint32_t* buf;        // LDS, kStage entries
-    unsigned* cnt;        // LDS counter
-    // stores the entry of each lane with `app`: `before` entries of the
-    // batch precede this ballot, the batch has `tot` entries and owns LDS
-    // slots [base, base + tot); slots beyond kStage go to the global list
-    __device__ __forceinline__ void append_wave(bool app, uint32_t v, uint64_t bal, unsigned before, unsigned tot,
-                                                unsigned base, uint32_t* tl, unsigned* tn, unsigned& gbase) const {
-        unsigned const pos = base + before + __popcll(bal & lanemask_lt());
-        unsigned const over_start = base > unsigned(kStage) ? base : unsigned(kStage);
-        if (app) {
-            if (pos < unsigned(kStage)) buf[pos] = v;
-            else ast(&tl[gbase + (pos - over_start)], v);
-        }
-        (void)tot;
-        (void)tn;
+// Largest j in [0, n) with a[j] <= q (a non-decreasing, a[0] <= q),
+// branch-free over LDS; N is a power of two >= n.
+template <int N>
+__device__ __forceinline__ uint32_t last_le(const uint32_t* a, uint32_t n, uint32_t q) {
+    uint32_t j = 0;
+#pragma unroll
+    for (uint32_t step = N / 2; step > 0; step >>= 1) {
+        uint32_t const k = j + step;
+        uint32_t const v = a[k < n ? k : n - 1];
+        j = (k < n && v <= q) ? k : j;
     }
-    // at the end of a phase, with the whole block
-    __device__ __forceinline__ void flush(uint32_t* tl, unsigned* tn, unsigned* s_tmp) const {
-        __syncthreads();
-        unsigned const n = *cnt < unsigned(kStage) ? *cnt : unsigned(kStage);
-        if (threadIdx.x == 0) *s_tmp = n ? afetch_add(tn, n) : 0u;
-        __syncthreads();
-        unsigned const gb = *s_tmp;
-        for (unsigned i = threadIdx.x; i < n; i += blockDim.x) ast(&tl[gb + i], buf[i]);
-        __syncthreads();
-        if (threadIdx.x == 0) *cnt = 0;
-        __syncthreads();
-    }
-};
+    return j;
+}
 
-// Claims, for sub-round X, every bloom that holds edge ee (the whole wave
-// works on the edge's incidence list); a bloom is claimed once per
-// sub-round (max on its claim stamp). All lanes of the wave must call.
-__device__ __forceinline__ void claim_edge(uint32_t ee, uint32_t X, const MspArgs& a, const Stage& st) {
-    int const lane = threadIdx.x & (kWave - 1);
-    uint32_t* tl = a.touched + size_t(X % 3) * a.nbloom;
-    unsigned* tn = &a.ctl->touched_n[X % 3];
-    uint32_t const p0 = a.inc_off[ee], p1 = a.inc_off[ee + 1];
-    for (uint32_t q = p0; q < p1; q += kWave * kClaimUnroll) {
-        uint32_t sb[kClaimUnroll];
-        bool app[kClaimUnroll];
-#pragma unroll
-        for (int u = 0; u < kClaimUnroll; u++) {
-            uint32_t const p = q + u * kWave + lane;
-            sb[u] = p < p1 ? a.inc[p] : 0;
-        }
-#pragma unroll
-        for (int u = 0; u < kClaimUnroll; u++) {
-            uint32_t const p = q + u * kWave + lane;
-            app[u] = p < p1 && afetch_max(&a.claim[sb[u]], X) < X;
-        }
-        uint64_t bal[kClaimUnroll];
+// Claim stamp index for a lane with no incidence in a claim step: one per
+// (block, thread, unroll slot) past the R real stamps.
+__device__ __forceinline__ uint32_t dummy_claim(const MspArgs& a, int u) {
+    return a.R + (blockIdx.x * kMB + threadIdx.x) * kClaimUnroll + uint32_t(u);
+}
+
+// Claim list of one sub-round Y being filled by this block: its own list
+// first, the shared list beyond kBlkCap.
+struct Out {
+    uint32_t* own;        // area + (Y % 3, B)
+    uint32_t* shared;     // ovf + Y % 3
+    unsigned* shared_n;   // ctl->ovf_n + Y % 3
+    unsigned* cnt;        // LDS counter
+    // appends, for one wave, the entries v[u] of the lanes with app[u]
+    template <int U>
+    __device__ __forceinline__ void append(const bool* app, const uint32_t* v) const {
+        int const lane = threadIdx.x & (kWave - 1);
+        uint64_t bal[U];
         unsigned tot = 0;
 #pragma unroll
-        for (int u = 0; u < kClaimUnroll; u++) {
+        for (int u = 0; u < U; u++) {
             bal[u] = __ballot(app[u]);
             tot += unsigned(__popcll(bal[u]));
         }
-        if (tot) {
-            // LDS slots for the batch; the part beyond kStage is reserved in
-            // the global list at once
-            unsigned base = 0, gbase = 0;
-            if (lane == 0) {
-                base = atomicAdd(st.cnt, tot);
-                unsigned const over_start = base > unsigned(kStage) ? base : unsigned(kStage);
-                unsigned const n_over = base + tot > unsigned(kStage) ? base + tot - over_start : 0u;
-                if (n_over) gbase = afetch_add(tn, n_over);
+        if (!tot) return;
+        unsigned base = 0, gbase = 0;
+        if (lane == 0) {
+            base = atomicAdd(cnt, tot);
+            unsigned const over_start = base > kBlkCap ? base : kBlkCap;
+            if (base + tot > kBlkCap) gbase = afetch_add(shared_n, base + tot - over_start);
+        }
+        base = __shfl(base, 0, kWave);
+        gbase = __shfl(gbase, 0, kWave);
+        unsigned const over_start = base > kBlkCap ? base : kBlkCap;
+        unsigned before = base;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            unsigned const pos = before + unsigned(__popcll(bal[u] & lanemask_lt()));
+            if (app[u]) {
+                if (pos < kBlkCap) ast(&own[pos], v[u]);
+                else ast(&shared[gbase + (pos - over_start)], v[u]);
             }
-            base = __shfl(base, 0, kWave);
-            gbase = __shfl(gbase, 0, kWave);
-            unsigned before = 0;
+            before += unsigned(__popcll(bal[u]));
+        }
+    }
+};
+
+// Per-wave LDS scratch for flattening the incidence lists of a batch of
+// up to 64 edges.
+struct WaveScan {
+    uint32_t* incl;  // inclusive prefix of list lengths
+    uint32_t* off;   // incidence index minus flattened position
+};
+
+// Claims, for sub-round Y, every bloom that holds one of the n edges in xl
+// (LDS), with one wave: used when a wave's frontier buffer fills up before
+// the block claims together. Branch-free, so that all loads and then all
+// atomics of a step are in flight together: lanes past the end claim a
+// private dummy stamp instead (the same address for all of them would
+// serialise their atomics) and append nothing. Of all the claims of one
+// bloom for sub-round Y exactly one finds the stamp below Y and appends it.
+__device__ __forceinline__ void claim_edges(const uint32_t* xl, uint32_t n, uint32_t Y, const MspArgs& a,
+                                            const Out& out, const WaveScan& ws) {
+    int const lane = threadIdx.x & (kWave - 1);
+    for (uint32_t b0 = 0; b0 < n; b0 += kWave) {
+        uint32_t const cnt = n - b0 < uint32_t(kWave) ? n - b0 : uint32_t(kWave);
+        uint32_t p0 = 0, len = 0;
+        if (uint32_t(lane) < cnt) {
+            uint32_t const e = xl[b0 + lane];
+            p0 = a.inc_off[e];
+            len = a.inc_off[e + 1] - p0;
+        }
+        uint32_t const incl = wave_incl_scan(len);
+        uint32_t const total = __shfl(incl, kWave - 1, kWave);
+        __builtin_amdgcn_wave_barrier();
+        ws.incl[lane] = incl;
+        ws.off[lane] = p0 - (incl - len);
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t q0 = 0; q0 < total; q0 += kWave * kClaimUnroll) {
+            uint32_t sb[kClaimUnroll], sc[kClaimUnroll];
 #pragma unroll
             for (int u = 0; u < kClaimUnroll; u++) {
-                st.append_wave(app[u], sb[u], bal[u], before, tot, base, tl, tn, gbase);
-                before += unsigned(__popcll(bal[u]));
+                uint32_t const qr = q0 + u * kWave + lane;
+                uint32_t const q = qr < total ? qr : total - 1;
+                // the first edge whose prefix passes q: one past the last <= q
+                uint32_t const j = ws.incl[0] > q ? 0u : last_le<kWave>(ws.incl, cnt, q) + 1;
+                sb[u] = a.inc[q + ws.off[j]];
+                sc[u] = qr < total ? sb[u] : dummy_claim(a, u);
             }
+            uint32_t old[kClaimUnroll];
+#pragma unroll
+            for (int u = 0; u < kClaimUnroll; u++) old[u] = afetch_max(&a.claim[sc[u]], Y);
+            bool app[kClaimUnroll];
+#pragma unroll
+            for (int u = 0; u < kClaimUnroll; u++) app[u] = q0 + u * kWave + lane < total && old[u] < Y;
+            out.append<kClaimUnroll>(app, sb);
         }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
-// Per-wave buffer of edges that fell to the level; their blooms are claimed
-// for the next sub-round after each bloom (or when the buffer fills).
+// Per-wave buffer of frontier edges of sub-round Y whose blooms the block
+// claims at the end of the phase; a full buffer is claimed by the wave.
 struct CrossBuf {
     uint32_t* xl;
     uint32_t n;
-    __device__ __forceinline__ void flush(uint32_t X, const MspArgs& a, const Stage& st) {
-        if (!n) return;
-        __builtin_amdgcn_wave_barrier();
-        for (uint32_t t = 0; t < n; t++) claim_edge(xl[t], X, a, st);
-        n = 0;
-    }
-    __device__ __forceinline__ void push(bool has, uint32_t e, uint32_t X, const MspArgs& a, const Stage& st) {
+    WaveScan ws;
+    __device__ __forceinline__ void push(bool has, uint32_t e, uint32_t Y, const MspArgs& a, const Out& out) {
         uint64_t const b = __ballot(has);
         uint32_t const cnt = __popcll(b);
         if (!cnt) return;
-        if (n + cnt > uint32_t(kXCap)) flush(X, a, st);
+        if (n + cnt > uint32_t(kXCap)) {
+            __builtin_amdgcn_wave_barrier();
+            claim_edges(xl, n, Y, a, out, ws);
+            n = 0;
+        }
         if (has) xl[n + __popcll(b & lanemask_lt())] = e;
         n += cnt;
     }
 };
+
+// The whole block claims, for sub-round Y, the blooms of the frontier edges
+// left in the waves' buffers (s_xl[w][0, s_xn[w])): their incidence lists
+// are concatenated (prefix in s_incl, index offset in s_off) and walked by
+// all lanes. All threads call, after a __syncthreads that follows the
+// waves' last pushes.
+__device__ __forceinline__ void claim_block(const MspArgs& a, uint32_t Y, const Out& out,
+                                            uint32_t (*s_xl)[kXCap], const uint32_t* s_xn, uint32_t* s_incl,
+                                            uint32_t* s_off, uint32_t* s_part) {
+    uint32_t const tid = threadIdx.x;
+    uint32_t pre[kWPB + 1];
+    pre[0] = 0;
+#pragma unroll
+    for (int w = 0; w < kWPB; w++) pre[w + 1] = pre[w] + s_xn[w];
+    uint32_t const n = pre[kWPB];
+    if (n == 0) return;
+    // thread t handles concatenated entries 2t and 2t + 1
+    uint32_t p0[2] = {0, 0}, len[2] = {0, 0};
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        uint32_t const j = 2 * tid + h;
+        if (j < n) {
+            int w = 0;
+#pragma unroll
+            for (int i = 1; i < kWPB; i++) w += j >= pre[i] ? 1 : 0;
+            uint32_t const e = s_xl[w][j - pre[w]];
+            p0[h] = a.inc_off[e];
+            len[h] = a.inc_off[e + 1] - p0[h];
+        }
+    }
+    uint32_t total = 0;
+    uint32_t const ex = block_excl_scan<kMB>(len[0] + len[1], s_part, total);
+    if (2 * tid < n) {
+        s_incl[2 * tid] = ex + len[0];
+        s_off[2 * tid] = p0[0] - ex;
+    }
+    if (2 * tid + 1 < n) {
+        s_incl[2 * tid + 1] = ex + len[0] + len[1];
+        s_off[2 * tid + 1] = p0[1] - (ex + len[0]);
+    }
+    __syncthreads();
+    for (uint32_t q0 = 0; q0 < total; q0 += kMB * kClaimUnroll) {
+        uint32_t sb[kClaimUnroll], sc[kClaimUnroll];
+#pragma unroll
+        for (int u = 0; u < kClaimUnroll; u++) {
+            uint32_t const qr = q0 + u * kMB + tid;
+            uint32_t const q = qr < total ? qr : total - 1;
+            uint32_t const j = s_incl[0] > q ? 0u : last_le<kBL>(s_incl, n, q) + 1;
+            sb[u] = a.inc[q + s_off[j]];
+            sc[u] = qr < total ? sb[u] : dummy_claim(a, u);
+        }
+        uint32_t old[kClaimUnroll];
+#pragma unroll
+        for (int u = 0; u < kClaimUnroll; u++) old[u] = afetch_max(&a.claim[sc[u]], Y);
+        bool app[kClaimUnroll];
+#pragma unroll
+        for (int u = 0; u < kClaimUnroll; u++) app[u] = q0 + u * kMB + tid < total && old[u] < Y;
+        out.append<kClaimUnroll>(app, sb);
+    }
+}
 
 struct Dec {
     uint64_t* sup;
@@ -330,9 +409,12 @@ struct Dec {
     uint64_t L;
     uint32_t X;
     uint64_t tmin;  // smallest decremented support still above the level
-    // subtracts d from the support of x; true when x falls to the level now
-    __device__ __forceinline__ bool operator()(uint32_t x, uint64_t d) {
-        uint64_t const old = afetch_add(&sup[x], uint64_t(0) - d);
+    // subtracts d from the support of x (atomic, returns the old support)
+    __device__ __forceinline__ uint64_t issue(uint32_t x, uint64_t d) const {
+        return afetch_add(&sup[x], uint64_t(0) - d);
+    }
+    // after issue: true when x fell to the level with this decrement
+    __device__ __forceinline__ bool settle(uint32_t x, uint64_t d, uint64_t old) {
         uint64_t const nw = old - d;
         if (old > L && nw <= L) {
             ast(&stamp[x], X + 1);
@@ -342,6 +424,7 @@ struct Dec {
         if (nw > L && nw < tmin) tmin = nw;
         return false;
     }
+    __device__ __forceinline__ bool operator()(uint32_t x, uint64_t d) { return settle(x, d, issue(x, d)); }
 };
 
 __device__ __forceinline__ int classify(uint32_t s1, uint32_t s2, uint32_t X) {
@@ -389,11 +472,11 @@ __device__ __forceinline__ uint64_t rule_scan(const MspArgs& a, const uint64_t* 
 template <int RPT>
 __device__ __forceinline__ void process_bloom(const MspArgs& a, uint32_t s, uint32_t kb, uint32_t up, uint64_t r0,
                                               uint32_t X, Dec& dec, uint64_t* c2, CrossBuf& xb,
-                                              const Stage& st) {
+                                              const Out& st) {
     int const lane = threadIdx.x & (kWave - 1);
     uint64_t* rec = a.rec + s;
     uint64_t r[RPT];
-    uint32_t st1[RPT];
+    uint32_t s1[RPT], s2[RPT];
     int c[RPT];
     r[0] = r0;
 #pragma unroll
@@ -401,16 +484,15 @@ __device__ __forceinline__ void process_bloom(const MspArgs& a, uint32_t s, uint
         uint32_t const j = i * kWave + lane;
         r[i] = j < kb ? ald(&rec[j]) : 0;
     }
+    // every stamp load is in flight before the first is used
 #pragma unroll
     for (int i = 0; i < RPT; i++) {
         uint32_t const j = i * kWave + lane;
-        c[i] = -1;
-        st1[i] = 0;
-        if (j < kb) {
-            st1[i] = ald(&a.stamp[uint32_t(r[i] >> 32)]);
-            c[i] = classify(st1[i], ald(&a.stamp[uint32_t(r[i])]), X);
-        }
+        s1[i] = j < kb ? ald(&a.stamp[uint32_t(r[i] >> 32)]) : 0u;
+        s2[i] = j < kb ? ald(&a.stamp[uint32_t(r[i])]) : 0u;
     }
+#pragma unroll
+    for (int i = 0; i < RPT; i++) c[i] = (i * kWave + lane) < int(kb) ? classify(s1[i], s2[i], X) : -1;
     uint32_t n0 = 0, n1 = 0, n2 = 0;
 #pragma unroll
     for (int i = 0; i < RPT; i++) {
@@ -424,26 +506,44 @@ __device__ __forceinline__ void process_bloom(const MspArgs& a, uint32_t s, uint
         n2 += __popcll(b2);
     }
     __builtin_amdgcn_wave_barrier();
-    uint64_t const dec0 = uint64_t(n1) + n2;
+    uint32_t const dec0 = n1 + n2;
     bool const c2ok = n2 <= uint32_t(kC2Cap);
+    // decrements: the first target is r-v (c = 0) or the survivor (c = 1),
+    // the second is v-w (c = 0 only, by dec0)
+    uint32_t t1[RPT], d1[RPT];
 #pragma unroll
     for (int i = 0; i < RPT; i++) {
         uint32_t const e1 = uint32_t(r[i] >> 32), e2 = uint32_t(r[i]);
-        bool x1 = false, x2 = false;
-        uint32_t f = e1;
-        if (c[i] == 0 && dec0) {
-            x1 = dec(e1, dec0);
-            x2 = dec(e2, dec0);
+        t1[i] = e1;
+        d1[i] = 0;
+        if (c[i] == 0) {
+            d1[i] = dec0;
         } else if (c[i] == 1) {
-            bool const froot = st1[i] != X;  // the survivor is r-v
-            f = froot ? e1 : e2;
+            bool const froot = s1[i] != X;  // the survivor is r-v
             uint32_t const g = froot ? e2 : e1;
-            uint64_t const d = uint64_t(n0) + (n1 - 1) +
-                               (c2ok ? rule_sum(c2, n2, froot, up, g) : rule_scan(a, rec, kb, X, froot, up, g));
-            if (d) x1 = dec(f, d);
+            t1[i] = froot ? e1 : e2;
+            d1[i] = n0 + (n1 - 1) +
+                    uint32_t(c2ok ? rule_sum(c2, n2, froot, up, g) : rule_scan(a, rec, kb, X, froot, up, g));
         }
-        xb.push(x1, f, X + 1, a, st);
-        xb.push(x2, e2, X + 1, a, st);
+    }
+    // the atomics of up to 8 rows are in flight before the first result is
+    // used (more would not fit the register budget)
+    constexpr int HB = RPT < 8 ? RPT : 8;
+#pragma unroll
+    for (int h = 0; h < RPT; h += HB) {
+        uint64_t o1[HB], o2[HB];
+#pragma unroll
+        for (int i = 0; i < HB; i++) {
+            o1[i] = d1[h + i] ? dec.issue(t1[h + i], d1[h + i]) : 0;
+            o2[i] = (c[h + i] == 0 && dec0) ? dec.issue(uint32_t(r[h + i]), dec0) : 0;
+        }
+#pragma unroll
+        for (int i = 0; i < HB; i++) {
+            bool const x1 = d1[h + i] && dec.settle(t1[h + i], d1[h + i], o1[i]);
+            bool const x2 = c[h + i] == 0 && dec0 && dec.settle(uint32_t(r[h + i]), dec0, o2[i]);
+            xb.push(x1, t1[h + i], X + 1, a, st);
+            xb.push(x2, uint32_t(r[h + i]), X + 1, a, st);
+        }
     }
     // compaction: only the middles without frontier edges stay
     if (n0 != kb) {
@@ -461,7 +561,7 @@ __device__ __forceinline__ void process_bloom(const MspArgs& a, uint32_t s, uint
 // Blooms above 1024 middles: a counting pass, then a decrement pass, then
 // (when the both-frontier list overflowed) a separate compaction pass.
 __device__ __forceinline__ void process_large(const MspArgs& a, uint32_t s, uint32_t kb, uint32_t up, uint32_t X, Dec& dec,
-                              uint64_t* c2, CrossBuf& xb, const Stage& st) {
+                              uint64_t* c2, CrossBuf& xb, const Out& st) {
     int const lane = threadIdx.x & (kWave - 1);
     uint64_t* rec = a.rec + s;
     uint32_t n0 = 0, n1 = 0, n2 = 0;
@@ -541,21 +641,71 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
     __shared__ uint64_t s_red[kWPB + 1];
     __shared__ uint64_t s_c2[kWPB][kC2Cap];
     __shared__ uint32_t s_xl[kWPB][kXCap];
-    __shared__ uint32_t s_stage[kStage];
-    __shared__ unsigned s_stage_n, s_tmp;
-    __shared__ Snap snap;
+    __shared__ uint32_t s_xn[kWPB];
+    __shared__ uint32_t s_ws[kWPB][2][kWave];
+    __shared__ uint32_t s_incl[kBL], s_off[kBL];
+    __shared__ uint32_t s_pfx[kMB + 1];
+    __shared__ unsigned s_on;
+    __shared__ int s_ok;
     uint32_t const G = gridDim.x, B = blockIdx.x, tid = threadIdx.x;
-    int const lane = tid & (kWave - 1), wv = tid / kWave;
+    int const wv = tid / kWave;
     uint32_t const gw = B * kWPB + wv, nw = G * kWPB;
     uint32_t const seg0 = uint32_t(uint64_t(a.m) * B / G), seg1 = uint32_t(uint64_t(a.m) * (B + 1) / G);
     MspCtl* ctl = a.ctl;
-    uint64_t* slot_m1 = a.slots + B;
-    uint64_t* slot_dec = a.slots + G + B;
-    if (tid == 0) s_stage_n = 0;
-    Stage const st{s_stage, &s_stage_n};
+    if (tid == 0) s_on = 0;
+    WaveScan const ws{s_ws[wv][0], s_ws[wv][1]};
+    unsigned epoch = 0;
+    // claim list of sub-round Y filled by this block
+    auto out_for = [&](uint32_t Y) {
+        return Out{a.area + (size_t(Y % 3) * G + B) * kBlkCap, a.ovf + size_t(Y % 3) * a.nbloom, &ctl->ovf_n[Y % 3],
+                   &s_on};
+    };
+    // publishes this block's claim-list length for sub-round Y (whole block)
+    auto close_out = [&](uint32_t Y) {
+        __syncthreads();
+        if (tid == 0) {
+            ast(&a.cnt[size_t(Y % 3) * G + B], s_on < kBlkCap ? s_on : kBlkCap);
+            s_on = 0;
+        }
+    };
+    // after a barrier: prefix of the blocks' list lengths for sub-round Y in
+    // s_pfx[0, G]; returns the number of claimed blooms (shared list included)
+    auto open_list = [&](uint32_t Y) -> uint32_t {
+        uint32_t const v = tid < G ? ald(&a.cnt[size_t(Y % 3) * G + tid]) : 0u;
+        uint32_t const sh = ald(&ctl->ovf_n[Y % 3]);
+        uint32_t total = 0;
+        uint32_t const ex = block_excl_scan<kMB>(v, s_part, total);
+        if (tid <= G) s_pfx[tid] = tid < G ? ex : total;
+        __syncthreads();
+        return total + sh;
+    };
+    // entry t of the claim list of sub-round Y (after open_list(Y))
+    auto list_entry = [&](uint32_t Y, uint32_t t) -> uint32_t {
+        uint32_t const own = s_pfx[G];
+        if (t >= own) return ald(&a.ovf[size_t(Y % 3) * a.nbloom + (t - own)]);
+        uint32_t const b = last_le<kMB>(s_pfx, G, t);
+        return ald(&a.area[(size_t(Y % 3) * G + b) * kBlkCap + (t - s_pfx[b])]);
+    };
+    // minimum over the blocks' slots of level parity p (whole block)
+    auto level_min = [&](uint32_t p) -> uint64_t {
+        const uint64_t* sl = a.slots + size_t(p) * 2 * G;
+        uint64_t v = kInf;
+        if (tid < G) {
+            uint64_t const x = ald(&sl[tid]), y = ald(&sl[G + tid]);
+            v = x < y ? x : y;
+        }
+        return block_min_all(v, s_red);
+    };
+    auto set_slots = [&](uint32_t p, uint64_t kept, uint64_t dec) {
+        if (tid == 0) {
+            ast(&a.slots[size_t(p) * 2 * G + B], kept);
+            ast(&a.slots[size_t(p) * 2 * G + G + B], dec);
+        }
+    };
 
     // ---- init: support-0 edges are final (process_support_zero); the rest
     // form this block's live segment; the first level is their min support
+    // (slots of parity 1, so that level 0 can write parity 0 at once)
     {
         uint32_t w = seg0;
         uint64_t mn = kInf;
@@ -579,135 +729,138 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
             w += tot;
         }
         if (tid == 0) a.live_cnt[B] = w - seg0;
-        uint64_t const bm = block_min_all(mn, s_red);
-        if (tid == 0) {
-            ast(slot_m1, (unsigned long long)bm);
-            ast(slot_dec, (unsigned long long)kInf);
-        }
+        set_slots(1, block_min_all(mn, s_red), kInf);
     }
-    if (!grid_sync(a, G, kSyncLevel, 2, &snap, s_red)) return;
-    uint64_t L = snap.next_level;
+    if (!grid_sync(a, G, ++epoch, &s_ok)) return;
+    uint64_t L = level_min(1);
     uint32_t X = 2, k = 0;
-    bool collect = true;
-    uint64_t tmin = kInf;       // this thread's smallest decremented support of the level
-    uint64_t dec_written = kInf;
-    uint32_t nt = 0;            // claimed blooms of the sub-round peeled next
+    uint32_t nt = 0;            // claimed blooms of sub-round X
     uint64_t t_col = 0, t_pe = 0, w_col = 0, w_pe = 0, pmax_sum = 0;
     while (L != kInf) {
-        uint64_t const tp0 = wall_clock64();
-        if (collect) {
-            // ---- collect: frontier of level L from this block's live segment
-            if (B == 0 && tid == 0) ast(&ctl->touched_n[(X + 1) % 3], 0u);
+        uint32_t const p = k & 1;
+        // ---- collect: frontier of level L from this block's live segment
+        {
+            uint64_t const tp0 = wall_clock64();
+            if (B == 0 && tid == 0) ast(&ctl->ovf_n[(X + 1) % 3], 0u);
+            Out const out = out_for(X);
             uint32_t const cnt = a.live_cnt[B];
-            __syncthreads();
             uint32_t w = seg0;
             uint64_t mn = kInf;
-            for (uint32_t base = seg0; base < seg0 + cnt; base += kMB) {
-                uint32_t const i = base + tid;
-                uint32_t e = 0;
-                bool keep = false, take = false;
-                if (i < seg0 + cnt) {
-                    // an edge finished at an earlier level has support below L
-                    // (it was peeled at a level < L with support <= that level);
-                    // a live edge has support >= L
-                    e = a.live[i];
-                    uint64_t const v = ald(&a.sup[e]);
-                    if (v == L) {
-                        take = true;
-                    } else if (v > L) {
-                        keep = true;
-                        mn = v < mn ? v : mn;
-                    }
+            CrossBuf xb{s_xl[wv], 0, ws};
+            constexpr int U = 4;
+            for (uint32_t base = seg0; base < seg0 + cnt; base += kMB * U) {
+                // an edge finished at an earlier level has support below L
+                // (it was peeled at a level < L with support <= that level);
+                // a live edge has support >= L
+                uint32_t e[U];
+                uint64_t v[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    uint32_t const i = base + u * kMB + tid;
+                    e[u] = i < seg0 + cnt ? a.live[i] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    uint32_t const i = base + u * kMB + tid;
+                    v[u] = i < seg0 + cnt ? ald(&a.sup[e[u]]) : 0ull;
+                }
+                uint32_t nk = 0;
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    uint32_t const i = base + u * kMB + tid;
+                    bool const in = i < seg0 + cnt;
+                    nk += (in && v[u] > L) ? 1u : 0u;
+                    if (in && v[u] > L) mn = v[u] < mn ? v[u] : mn;
                 }
                 uint32_t tot = 0;
-                uint32_t const ex = block_excl_scan<kMB>(keep ? 1u : 0u, s_part, tot);
-                if (keep) a.live[w + ex] = e;
+                uint32_t pos = w + block_excl_scan<kMB>(nk, s_part, tot);
                 w += tot;
-                if (take) {
-                    ast(&a.stamp[e], X);
-                    a.res[e] = L;
-                }
-                uint64_t pend = __ballot(take);
-                while (pend) {
-                    int const src = __ffsll((long long)pend) - 1;
-                    pend &= pend - 1;
-                    claim_edge(__shfl(e, src, kWave), X, a, st);
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    uint32_t const i = base + u * kMB + tid;
+                    bool const in = i < seg0 + cnt;
+                    if (in && v[u] > L) a.live[pos++] = e[u];
+                    bool const take = in && v[u] == L;
+                    if (take) {
+                        ast(&a.stamp[e[u]], X);
+                        a.res[e[u]] = L;
+                    }
+                    xb.push(take, e[u], X, a, out);
                 }
             }
             if (tid == 0) a.live_cnt[B] = w - seg0;
-            uint64_t const bm = block_min_all(mn, s_red);
-            if (tid == 0) {
-                ast(slot_m1, (unsigned long long)bm);
-                ast(slot_dec, (unsigned long long)kInf);
-            }
-            tmin = kInf;
-            dec_written = kInf;
-            st.flush(a.touched + size_t(X % 3) * a.nbloom, &ctl->touched_n[X % 3], &s_tmp);
-        } else {
-            // ---- peel sub-round X: every claimed bloom once
-            if (B == 0 && tid == 0) ast(&ctl->touched_n[(X + 2) % 3], 0u);
-            const uint32_t* tl = a.touched + size_t(X % 3) * a.nbloom;
+            set_slots(p, block_min_all(mn, s_red), kInf);
+            if ((tid & (kWave - 1)) == 0) s_xn[wv] = xb.n;
+            __syncthreads();
+            claim_block(a, X, out, s_xl, s_xn, s_incl, s_off, s_part);
+            close_out(X);
+            uint64_t const tp1 = wall_clock64();
+            if (!grid_sync(a, G, ++epoch, &s_ok)) return;
+            nt = open_list(X);
+            uint64_t const tp2 = wall_clock64();
+            t_col += tp2 - tp0;
+            w_col += tp1 - tp0;
+        }
+        // ---- peel sub-rounds X, X + 1, ... until one claims nothing
+        uint64_t tmin = kInf;   // this thread's smallest decremented support of the level
+        uint64_t dec_written = kInf;
+        for (;;) {
+            uint64_t const tp0 = wall_clock64();
+            if (B == 0 && tid == 0) ast(&ctl->ovf_n[(X + 2) % 3], 0u);
+            Out const out = out_for(X + 1);
             Dec dec{a.sup, a.stamp, a.res, L, X, tmin};
             uint64_t* c2 = s_c2[wv];
-            CrossBuf xb{s_xl[wv], 0};
+            CrossBuf xb{s_xl[wv], 0, ws};
+            int const lane = tid & (kWave - 1);
             uint64_t const tw0 = wall_clock64();
             for (uint32_t ti = gw; ti < nt; ti += nw) {
-                uint32_t const s = ald(&tl[ti]);
+                uint32_t const s = list_entry(X, ti);
                 // the header and the first 64 middles are loaded together
                 uint32_t const hdr = ald(&a.hdr[s]);
                 uint64_t const r0 = (s + lane < a.R) ? ald(&a.rec[s + lane]) : 0;
                 uint32_t const kb = hdr & 0x7FFFFFFFu, up = hdr >> 31;
                 if (kb < 2) continue;
-                if (kb <= uint32_t(kWave)) process_bloom<1>(a, s, kb, up, r0, X, dec, c2, xb, st);
-                else if (kb <= uint32_t(4 * kWave)) process_bloom<4>(a, s, kb, up, r0, X, dec, c2, xb, st);
-                else if (kb <= uint32_t(16 * kWave)) process_bloom<16>(a, s, kb, up, r0, X, dec, c2, xb, st);
-                else process_large(a, s, kb, up, X, dec, c2, xb, st);
-                xb.flush(X + 1, a, st);
+                if (kb <= uint32_t(kWave)) process_bloom<1>(a, s, kb, up, r0, X, dec, c2, xb, out);
+                else if (kb <= uint32_t(4 * kWave)) process_bloom<4>(a, s, kb, up, r0, X, dec, c2, xb, out);
+                else if (kb <= uint32_t(16 * kWave)) process_bloom<16>(a, s, kb, up, r0, X, dec, c2, xb, out);
+                else process_large(a, s, kb, up, X, dec, c2, xb, out);
             }
             tmin = dec.tmin;
             uint64_t const tw = wall_clock64() - tw0;
-            st.flush(a.touched + size_t((X + 1) % 3) * a.nbloom, &ctl->touched_n[(X + 1) % 3], &s_tmp);
+            if (lane == 0) s_xn[wv] = xb.n;
             // publish this block's smallest decremented support of the level
             uint64_t const bdec = block_min_all(tmin, s_red);
-            if (tid == 0 && bdec < dec_written) ast(slot_dec, (unsigned long long)bdec);
+            if (tid == 0 && bdec < dec_written) ast(&a.slots[size_t(p) * 2 * G + G + B], bdec);
             dec_written = bdec < dec_written ? bdec : dec_written;
+            claim_block(a, X + 1, out, s_xl, s_xn, s_incl, s_off, s_part);
+            close_out(X + 1);
             if (a.diag) {
                 uint64_t const mx = ~block_min_all(~tw, s_red);  // slowest wave of the block
                 if (tid == 0) afetch_max(&ctl->pmax[X & 1], (unsigned long long)mx);
             }
-        }
-        uint64_t const tp1 = wall_clock64();
-        // a collect fills the claim list of X; a peel fills that of X + 1
-        unsigned const idx = collect ? X % 3 : (X + 1) % 3;
-        if (!grid_sync(a, G, collect ? kSyncPlain : kSyncLevelIfIdle, idx, &snap, s_red)) return;
-        uint64_t const tp2 = wall_clock64();
-        nt = snap.touched;
-        if (collect) {
-            t_col += tp2 - tp0;
-            w_col += tp1 - tp0;
-            collect = false;
-            continue;
-        }
-        t_pe += tp2 - tp0;
-        w_pe += tp1 - tp0;
-        if (a.diag && B == 0 && tid == 0) {
-            unsigned long long const pm = ald(&ctl->pmax[X & 1]);
-            ast(&ctl->pmax[X & 1], 0ull);
-            pmax_sum += pm;
-            int lb = 0;
-            while (lb < 23 && (pm >> (lb + 1))) lb++;
-            ctl->hist[lb]++;
-            if (a.trace && X < a.trace_cap) a.trace[X] = (uint64_t(nt) << 32) | (pm & 0xFFFFFFFFull);
-        }
-        if (nt) {
+            uint64_t const tp1 = wall_clock64();
+            if (!grid_sync(a, G, ++epoch, &s_ok)) return;
+            uint32_t const nn = open_list(X + 1);
+            uint64_t const tp2 = wall_clock64();
+            t_pe += tp2 - tp0;
+            w_pe += tp1 - tp0;
+            if (a.diag && B == 0 && tid == 0) {
+                unsigned long long const pm = ald(&ctl->pmax[X & 1]);
+                ast(&ctl->pmax[X & 1], 0ull);
+                pmax_sum += pm;
+                int lb = 0;
+                while (lb < 23 && (pm >> (lb + 1))) lb++;
+                ctl->hist[lb]++;
+                if (a.trace && X < a.trace_cap) a.trace[X] = (uint64_t(nn) << 32) | (pm & 0xFFFFFFFFull);
+            }
+            if (!nn) break;
+            nt = nn;
             X++;
-            continue;
         }
         // no bloom claimed for X + 1: the level is over (X + 1 is skipped)
-        L = snap.next_level;
+        L = level_min(p);
         X += 2;
         k++;
-        collect = true;
     }
     if (B == 0 && tid == 0) {
         ctl->subrounds = X - 1;
@@ -720,11 +873,12 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
     }
 }
 
-__global__ void k_msp_ctl_init(MspCtl* ctl) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
+__global__ void k_msp_init(MspCtl* ctl, uint32_t* arrive, uint32_t n_arrive) {
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_arrive) arrive[i] = 0;
+    if (i == 0) {
         unsigned long long* w = reinterpret_cast<unsigned long long*>(ctl);
-        for (size_t i = 0; i < sizeof(MspCtl) / sizeof(unsigned long long); i++) w[i] = 0;
-        ctl->next_level = kInf;
+        for (size_t j = 0; j < sizeof(MspCtl) / sizeof(unsigned long long); j++) w[j] = 0;
     }
 }
 
@@ -751,19 +905,33 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     BT_TRY(bt_index_build(dg));
     BT_TRY(tm.start(6));
     bt_index& ix = c->ix;
-    unsigned const G = unsigned(c->cu_count);  // one block per CU keeps the grid barrier short
+    // one block per CU keeps the grid barrier short; block 0 watches every
+    // other block with one thread
+    unsigned const G = unsigned(c->cu_count);
+    if (G == 0 || G >= unsigned(kMB)) return BT_ERR_DEVICE;
     int per_cu = 0;
     BT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_msp_persist),
                                                             kMB, 0));
     if (per_cu < 1) return BT_ERR_DEVICE;
+    uint32_t const nbloom = uint32_t(std::max<uint64_t>(ix.NB, 1));
+    // claim stamps: R real ones, then the dummies (both sized by the index build)
+    if (ix.bclaim.bytes < sizeof(uint32_t) * (ix.R + size_t(G) * kClaimDummiesPerBlock)) return BT_ERR_INTERNAL;
     BT_TRY(bt_buf_reserve(ix.live, sizeof(uint32_t) * std::max<size_t>(m, 1)));
-    size_t const cnt_bytes = (sizeof(uint32_t) * (G + 1) + 7) & ~size_t(7);
-    BT_TRY(bt_buf_reserve(ix.seg, cnt_bytes + sizeof(uint64_t) * 2 * G));
+    BT_TRY(bt_buf_reserve(ix.area, sizeof(uint32_t) * 3 * size_t(G) * kBlkCap));
+    BT_TRY(bt_buf_reserve(ix.touched, sizeof(uint32_t) * 3 * size_t(nbloom)));
+    // per-block words: live counts, list lengths, level slots, barrier slots
+    auto up256 = [](size_t b) { return (b + 255) & ~size_t(255); };
+    size_t const o_cnt = up256(sizeof(uint32_t) * G);
+    size_t const o_slots = o_cnt + up256(sizeof(uint32_t) * 3 * G);
+    size_t const o_arrive = o_slots + up256(sizeof(uint64_t) * 4 * G);
+    size_t const n_arrive = size_t(G) * kSp;
+    BT_TRY(bt_buf_reserve(ix.seg, o_arrive + sizeof(uint32_t) * n_arrive));
+    char* seg = static_cast<char*>(ix.seg.ptr);
     MspCtl* ctl = reinterpret_cast<MspCtl*>(dg->ctl);
     MspArgs a{};
     a.m = m;
-    a.nbloom = uint32_t(std::max<uint64_t>(ix.NB, 1));
     a.R = uint32_t(ix.R);
+    a.nbloom = nbloom;
     a.sup = dg->sup;
     a.res = dg->res;
     a.stamp = dg->list0;
@@ -772,10 +940,13 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     a.rec = static_cast<uint64_t*>(ix.vals1.ptr);
     a.hdr = static_cast<uint32_t*>(ix.bk.ptr);
     a.claim = static_cast<uint32_t*>(ix.bclaim.ptr);
-    a.touched = static_cast<uint32_t*>(ix.touched.ptr);
+    a.area = static_cast<uint32_t*>(ix.area.ptr);
+    a.ovf = static_cast<uint32_t*>(ix.touched.ptr);
     a.live = static_cast<uint32_t*>(ix.live.ptr);
-    a.live_cnt = static_cast<uint32_t*>(ix.seg.ptr);
-    a.slots = reinterpret_cast<uint64_t*>(static_cast<char*>(ix.seg.ptr) + cnt_bytes);
+    a.live_cnt = reinterpret_cast<uint32_t*>(seg);
+    a.cnt = reinterpret_cast<uint32_t*>(seg + o_cnt);
+    a.slots = reinterpret_cast<uint64_t*>(seg + o_slots);
+    a.arrive = reinterpret_cast<uint32_t*>(seg + o_arrive);
     a.ctl = ctl;
     static bt_buf trace_buf;
     bool const tracing = getenv("BT_PEEL_TRACE_FILE") != nullptr;
@@ -786,7 +957,7 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
         a.trace = static_cast<uint64_t*>(trace_buf.ptr);
         BT_HIP_TRY(hipMemsetAsync(a.trace, 0, sizeof(uint64_t) * a.trace_cap, s));
     }
-    k_msp_ctl_init<<<1, 64, 0, s>>>(ctl);
+    k_msp_init<<<grid_for(n_arrive, 256, 1u << 16), 256, 0, s>>>(ctl, a.arrive, uint32_t(n_arrive));
     BT_HIP_TRY(hipGetLastError());
     void* params[] = {&a};
     BT_HIP_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_msp_persist), dim3(G), dim3(kMB), params,

@@ -152,7 +152,7 @@ int bt_ctx_destroy(bt_ctx* c) {
     bt_buf_free(c->pool);
     bt_buf_free(c->pinned_dummy);
     bt_buf* ixb[] = {&c->ix.keys0, &c->ix.keys1, &c->ix.vals0, &c->ix.vals1, &c->ix.bid, &c->ix.bstart,
-                     &c->ix.bk, &c->ix.bclaim, &c->ix.inc_off, &c->ix.inc_cur, &c->ix.inc, &c->ix.touched,
+                     &c->ix.bk, &c->ix.bclaim, &c->ix.inc_off, &c->ix.inc_cur, &c->ix.inc, &c->ix.touched, &c->ix.area,
                      &c->ix.live, &c->ix.seg};
     for (bt_buf* b : ixb) bt_buf_free(*b);
     for (auto& e : c->ev)
