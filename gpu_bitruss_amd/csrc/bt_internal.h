@@ -101,7 +101,7 @@ struct bt_dgraph {
     uint8_t* state = nullptr;   // m   : msp edge state
     uint32_t* list0 = nullptr;  // m   : frontier buffers / root lists
     uint32_t* list1 = nullptr;  // m
-    uint64_t* ctl = nullptr;    // 64 control words
+    uint64_t* ctl = nullptr;    // 128 control words
     uint64_t* r_key = nullptr;  // 2m  : sort keys (prep only)
     uint64_t* r_key2 = nullptr; // 2m
     uint32_t* r_val2 = nullptr; // 2m

@@ -66,8 +66,8 @@ constexpr int kWPB = kMB / kWave;       // waves per block
 constexpr int kC2Cap = 128;             // both-frontier middles kept in LDS per wave
 constexpr int kClaimUnroll = 4;         // incidences per lane per claim step
 static_assert(kMB * kClaimUnroll == int(kClaimDummiesPerBlock), "dummy claim stamps per block");
-constexpr int kXCap = 128;              // frontier edges buffered per wave
-constexpr int kBL = kWPB * kXCap;       // frontier edges claimed together by a block
+constexpr int kXCap = 2048;             // frontier edges buffered per wave
+constexpr int kBL = 1024;               // frontier edges per block claim batch
 constexpr uint32_t kBlkCap = 16384;     // claimed blooms a block keeps in its own list
 constexpr int kSp = 32;                 // u32 spacing of the barrier slots (128 B)
 constexpr uint64_t kInf = ~0ull;
@@ -87,7 +87,7 @@ struct MspCtl {
     unsigned long long pmax[2], pmax_sum;
     unsigned long long hist[24];
 };
-static_assert(sizeof(MspCtl) <= 64 * sizeof(uint64_t), "control block overlays dg->ctl");
+static_assert(sizeof(MspCtl) <= 128 * sizeof(uint64_t), "control block overlays dg->ctl");
 
 struct MspArgs {
     uint32_t m;
@@ -323,22 +323,26 @@ __device__ __forceinline__ void claim_edges(const uint32_t* xl, uint32_t n, uint
 }
 
 // Per-wave buffer of frontier edges of sub-round Y whose blooms the block
-// claims at the end of the phase; a full buffer is claimed by the wave.
+// claims at the end of the phase. Room is made before a batch of puts (the
+// wave claims a buffer that could overflow itself), so that the puts carry
+// no claim code: every inlined copy of it would grow the kernel, whose
+// instruction fetches are a cost of their own.
 struct CrossBuf {
     uint32_t* xl;
     uint32_t n;
     WaveScan ws;
-    __device__ __forceinline__ void push(bool has, uint32_t e, uint32_t Y, const MspArgs& a, const Out& out) {
-        uint64_t const b = __ballot(has);
-        uint32_t const cnt = __popcll(b);
-        if (!cnt) return;
-        if (n + cnt > uint32_t(kXCap)) {
+    // after this, `room` more edges fit (room <= kXCap)
+    __device__ __forceinline__ void ensure(uint32_t room, uint32_t Y, const MspArgs& a, const Out& out) {
+        if (n + room > uint32_t(kXCap)) {
             __builtin_amdgcn_wave_barrier();
             claim_edges(xl, n, Y, a, out, ws);
             n = 0;
         }
+    }
+    __device__ __forceinline__ void put(bool has, uint32_t e) {
+        uint64_t const b = __ballot(has);
         if (has) xl[n + __popcll(b & lanemask_lt())] = e;
-        n += cnt;
+        n += __popcll(b);
     }
 };
 
@@ -355,50 +359,54 @@ __device__ __forceinline__ void claim_block(const MspArgs& a, uint32_t Y, const 
     pre[0] = 0;
 #pragma unroll
     for (int w = 0; w < kWPB; w++) pre[w + 1] = pre[w] + s_xn[w];
-    uint32_t const n = pre[kWPB];
-    if (n == 0) return;
-    // thread t handles concatenated entries 2t and 2t + 1
-    uint32_t p0[2] = {0, 0}, len[2] = {0, 0};
+    uint32_t const n_all = pre[kWPB];
+    // batches of kBL edges; thread t handles entries 2t and 2t + 1 of a batch
+    for (uint32_t b0 = 0; b0 < n_all; b0 += kBL) {
+        uint32_t const n = n_all - b0 < uint32_t(kBL) ? n_all - b0 : uint32_t(kBL);
+        uint32_t p0[2] = {0, 0}, len[2] = {0, 0};
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-        uint32_t const j = 2 * tid + h;
-        if (j < n) {
-            int w = 0;
+        for (int h = 0; h < 2; h++) {
+            uint32_t const j = 2 * tid + h;
+            if (j < n) {
+                uint32_t const jj = b0 + j;
+                int w = 0;
 #pragma unroll
-            for (int i = 1; i < kWPB; i++) w += j >= pre[i] ? 1 : 0;
-            uint32_t const e = s_xl[w][j - pre[w]];
-            p0[h] = a.inc_off[e];
-            len[h] = a.inc_off[e + 1] - p0[h];
+                for (int i = 1; i < kWPB; i++) w += jj >= pre[i] ? 1 : 0;
+                uint32_t const e = s_xl[w][jj - pre[w]];
+                p0[h] = a.inc_off[e];
+                len[h] = a.inc_off[e + 1] - p0[h];
+            }
         }
-    }
-    uint32_t total = 0;
-    uint32_t const ex = block_excl_scan<kMB>(len[0] + len[1], s_part, total);
-    if (2 * tid < n) {
-        s_incl[2 * tid] = ex + len[0];
-        s_off[2 * tid] = p0[0] - ex;
-    }
-    if (2 * tid + 1 < n) {
-        s_incl[2 * tid + 1] = ex + len[0] + len[1];
-        s_off[2 * tid + 1] = p0[1] - (ex + len[0]);
-    }
-    __syncthreads();
-    for (uint32_t q0 = 0; q0 < total; q0 += kMB * kClaimUnroll) {
-        uint32_t sb[kClaimUnroll], sc[kClaimUnroll];
-#pragma unroll
-        for (int u = 0; u < kClaimUnroll; u++) {
-            uint32_t const qr = q0 + u * kMB + tid;
-            uint32_t const q = qr < total ? qr : total - 1;
-            uint32_t const j = s_incl[0] > q ? 0u : last_le<kBL>(s_incl, n, q) + 1;
-            sb[u] = a.inc[q + s_off[j]];
-            sc[u] = qr < total ? sb[u] : dummy_claim(a, u);
+        uint32_t total = 0;
+        uint32_t const ex = block_excl_scan<kMB>(len[0] + len[1], s_part, total);
+        if (2 * tid < n) {
+            s_incl[2 * tid] = ex + len[0];
+            s_off[2 * tid] = p0[0] - ex;
         }
-        uint32_t old[kClaimUnroll];
+        if (2 * tid + 1 < n) {
+            s_incl[2 * tid + 1] = ex + len[0] + len[1];
+            s_off[2 * tid + 1] = p0[1] - (ex + len[0]);
+        }
+        __syncthreads();
+        for (uint32_t q0 = 0; q0 < total; q0 += kMB * kClaimUnroll) {
+            uint32_t sb[kClaimUnroll], sc[kClaimUnroll];
 #pragma unroll
-        for (int u = 0; u < kClaimUnroll; u++) old[u] = afetch_max(&a.claim[sc[u]], Y);
-        bool app[kClaimUnroll];
+            for (int u = 0; u < kClaimUnroll; u++) {
+                uint32_t const qr = q0 + u * kMB + tid;
+                uint32_t const q = qr < total ? qr : total - 1;
+                uint32_t const j = s_incl[0] > q ? 0u : last_le<kBL>(s_incl, n, q) + 1;
+                sb[u] = a.inc[q + s_off[j]];
+                sc[u] = qr < total ? sb[u] : dummy_claim(a, u);
+            }
+            uint32_t old[kClaimUnroll];
 #pragma unroll
-        for (int u = 0; u < kClaimUnroll; u++) app[u] = q0 + u * kMB + tid < total && old[u] < Y;
-        out.append<kClaimUnroll>(app, sb);
+            for (int u = 0; u < kClaimUnroll; u++) old[u] = afetch_max(&a.claim[sc[u]], Y);
+            bool app[kClaimUnroll];
+#pragma unroll
+            for (int u = 0; u < kClaimUnroll; u++) app[u] = q0 + u * kMB + tid < total && old[u] < Y;
+            out.append<kClaimUnroll>(app, sb);
+        }
+        __syncthreads();
     }
 }
 
@@ -526,9 +534,10 @@ __device__ __forceinline__ void process_bloom(const MspArgs& a, uint32_t s, uint
                     uint32_t(c2ok ? rule_sum(c2, n2, froot, up, g) : rule_scan(a, rec, kb, X, froot, up, g));
         }
     }
-    // the atomics of up to 8 rows are in flight before the first result is
-    // used (more would not fit the register budget)
-    constexpr int HB = RPT < 8 ? RPT : 8;
+    // the atomics of several rows (8, or 4 in the largest tier) are in
+    // flight before the first result is used; more would not fit the
+    // register budget
+    constexpr int HB = RPT < 8 ? RPT : (RPT > 8 ? 4 : 8);
 #pragma unroll
     for (int h = 0; h < RPT; h += HB) {
         uint64_t o1[HB], o2[HB];
@@ -541,8 +550,8 @@ __device__ __forceinline__ void process_bloom(const MspArgs& a, uint32_t s, uint
         for (int i = 0; i < HB; i++) {
             bool const x1 = d1[h + i] && dec.settle(t1[h + i], d1[h + i], o1[i]);
             bool const x2 = c[h + i] == 0 && dec0 && dec.settle(uint32_t(r[h + i]), dec0, o2[i]);
-            xb.push(x1, t1[h + i], X + 1, a, st);
-            xb.push(x2, uint32_t(r[h + i]), X + 1, a, st);
+            xb.put(x1, t1[h + i]);
+            xb.put(x2, uint32_t(r[h + i]));
         }
     }
     // compaction: only the middles without frontier edges stay
@@ -554,7 +563,12 @@ __device__ __forceinline__ void process_bloom(const MspArgs& a, uint32_t s, uint
             if (c[i] == 0) ast(&rec[wpos + __popcll(keepm & lanemask_lt())], r[i]);
             wpos += __popcll(keepm);
         }
-        if (lane == 0) ast(&a.hdr[s], n0 | (up << 31));
+        if (lane == 0) {
+            ast(&a.hdr[s], n0 | (up << 31));
+            // fewer than two middles left: no butterfly remains, and the
+            // top claim stamp keeps the bloom out of every later claim list
+            if (n0 < 2) afetch_max(&a.claim[s], ~0u);
+        }
     }
 }
 
@@ -587,6 +601,7 @@ __device__ __forceinline__ void process_large(const MspArgs& a, uint32_t s, uint
     bool const inline_compact = n2 <= uint32_t(kC2Cap);
     uint32_t wpos = 0;
     for (uint32_t j0 = 0; j0 < kb; j0 += kWave) {
+        xb.ensure(2 * kWave, X + 1, a, st);
         uint32_t const j = j0 + lane;
         int c = -1;
         uint64_t r = 0;
@@ -616,8 +631,8 @@ __device__ __forceinline__ void process_large(const MspArgs& a, uint32_t s, uint
             if (c == 0) ast(&rec[wpos + __popcll(keepm & lanemask_lt())], r);
             wpos += __popcll(keepm);
         }
-        xb.push(x1, f, X + 1, a, st);
-        xb.push(x2, e2, X + 1, a, st);
+        xb.put(x1, f);
+        xb.put(x2, e2);
     }
     if (!inline_compact) {
         for (uint32_t j0 = 0; j0 < kb; j0 += kWave) {
@@ -633,7 +648,10 @@ __device__ __forceinline__ void process_large(const MspArgs& a, uint32_t s, uint
             wpos += __popcll(keepm);
         }
     }
-    if (lane == 0) ast(&a.hdr[s], wpos | (up << 31));
+    if (lane == 0) {
+        ast(&a.hdr[s], wpos | (up << 31));
+        if (wpos < 2) afetch_max(&a.claim[s], ~0u);
+    }
 }
 
 __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
@@ -749,6 +767,7 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
             CrossBuf xb{s_xl[wv], 0, ws};
             constexpr int U = 4;
             for (uint32_t base = seg0; base < seg0 + cnt; base += kMB * U) {
+                xb.ensure(U * kWave, X, a, out);
                 // an edge finished at an earlier level has support below L
                 // (it was peeled at a level < L with support <= that level);
                 // a live edge has support >= L
@@ -785,7 +804,7 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
                         ast(&a.stamp[e[u]], X);
                         a.res[e[u]] = L;
                     }
-                    xb.push(take, e[u], X, a, out);
+                    xb.put(take, e[u]);
                 }
             }
             if (tid == 0) a.live_cnt[B] = w - seg0;
@@ -813,13 +832,30 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
             CrossBuf xb{s_xl[wv], 0, ws};
             int const lane = tid & (kWave - 1);
             uint64_t const tw0 = wall_clock64();
+            // software pipeline over this wave's blooms: the list entry two
+            // blooms ahead and the header and first 64 middles of the next
+            // one load while the current bloom is processed
+            uint32_t s_next = 0, s_next2 = 0, hdr_next = 0;
+            uint64_t r0_next = 0;
+            if (gw < nt) {
+                s_next = list_entry(X, gw);
+                hdr_next = ald(&a.hdr[s_next]);
+                r0_next = (s_next + lane < a.R) ? ald(&a.rec[s_next + lane]) : 0;
+            }
+            if (gw + nw < nt) s_next2 = list_entry(X, gw + nw);
             for (uint32_t ti = gw; ti < nt; ti += nw) {
-                uint32_t const s = list_entry(X, ti);
-                // the header and the first 64 middles are loaded together
-                uint32_t const hdr = ald(&a.hdr[s]);
-                uint64_t const r0 = (s + lane < a.R) ? ald(&a.rec[s + lane]) : 0;
+                uint32_t const s = s_next, hdr = hdr_next;
+                uint64_t const r0 = r0_next;
+                if (ti + nw < nt) {
+                    s_next = s_next2;
+                    hdr_next = ald(&a.hdr[s_next]);
+                    r0_next = (s_next + lane < a.R) ? ald(&a.rec[s_next + lane]) : 0;
+                    if (ti + 2 * nw < nt) s_next2 = list_entry(X, ti + 2 * nw);
+                }
                 uint32_t const kb = hdr & 0x7FFFFFFFu, up = hdr >> 31;
                 if (kb < 2) continue;
+                // a bloom frees at most two edges per middle
+                xb.ensure(2 * (kb < uint32_t(16 * kWave) ? kb : uint32_t(16 * kWave)), X + 1, a, out);
                 if (kb <= uint32_t(kWave)) process_bloom<1>(a, s, kb, up, r0, X, dec, c2, xb, out);
                 else if (kb <= uint32_t(4 * kWave)) process_bloom<4>(a, s, kb, up, r0, X, dec, c2, xb, out);
                 else if (kb <= uint32_t(16 * kWave)) process_bloom<16>(a, s, kb, up, r0, X, dec, c2, xb, out);

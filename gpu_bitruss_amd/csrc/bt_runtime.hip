@@ -239,7 +239,7 @@ int bt_graph_upload(bt_ctx* ctx, const bt_host_graph* g, bt_dgraph** out) {
     A(dalloc(&dg->state, m));
     A(dalloc(&dg->list0, m + n + 1));
     A(dalloc(&dg->list1, m + n + 1));
-    A(dalloc(&dg->ctl, 64));
+    A(dalloc(&dg->ctl, 128));
     A(dalloc(&dg->r_key, 2 * m + n));
     A(dalloc(&dg->r_key2, 2 * m + n));
     A(dalloc(&dg->r_val2, 2 * m + n));
