@@ -260,7 +260,6 @@ __global__ __launch_bounds__(kPeelBlock) void k_hidx_round(uint32_t m, Csr g, co
             sh.width = (pe + kHBins) / kHBins;  // ceil((pe + 1) / kHBins)
         }
         __syncthreads();
-        bool first = true;
         while (true) {
             uint64_t const lo = sh.lo, hi = sh.hi, width = sh.width;
             for (int i = threadIdx.x; i < kHBins; i += kPeelBlock) sh.hist[i] = 0;
@@ -272,11 +271,10 @@ __global__ __launch_bounds__(kPeelBlock) void k_hidx_round(uint32_t m, Csr g, co
                 uint64_t val = prev[wv];
                 val = a < val ? a : val;
                 val = b < val ? b : val;
-                if (first) {
-                    if (val > hi) val = hi;
-                } else if (val > hi) {
-                    return;  // already counted in `above`
-                }
+                // clipped to prev[e] (a bound on the answer) in every pass,
+                // so a value above prev[e] counts at prev[e] in the top bin
+                if (val > pe) val = pe;
+                if (val > hi) return;  // already counted in `above`
                 if (val < lo) return;
                 atomicAdd(&sh.hist[(val - lo) / width], 1u);
             };
@@ -309,7 +307,6 @@ __global__ __launch_bounds__(kPeelBlock) void k_hidx_round(uint32_t m, Csr g, co
                 }
             }
             __syncthreads();
-            first = false;
             if (sh.done) break;
         }
         if (threadIdx.x == 0) {
@@ -413,7 +410,11 @@ int bt_hidx_impl(bt_dgraph* dg, uint32_t* rounds_out) {
     bt_ctx* c = dg->ctx;
     hipStream_t s = c->stream;
     uint32_t const m = dg->m;
-    BT_TRY(bt_count_impl(dg, true, nullptr, nullptr));
+    // the bloom index when it fits (the path option applies to hidx too)
+    bool const want_index = c->msp_path != BT_MSP_PATH_MERGE;
+    BT_TRY(bt_count_impl(dg, true, nullptr, nullptr, want_index));
+    if (want_index && c->ix.R != kNoIndex) return bt_hidx_index(dg, rounds_out);
+    if (c->msp_path == BT_MSP_PATH_INDEX) return BT_ERR_BUDGET;
     bt_timings const count_t = c->last;
     bt_timer tm(c);
     BT_TRY(tm.start(5));

@@ -1,0 +1,246 @@
+// h-index bitruss numbers (Jacobi rounds) over the bloom index.
+//
+// Reference semantics: g_bitruss_hindex / bitruss_decomposition_h_index
+// (bitruss_hidx.cu:51-256) and c_bitruss_hindex / compute_bitruss_hindex
+// (cpu_bitruss.cpp:125-239): start from the butterfly supports; in each round
+// every edge e with prev[e] > 0 takes the h-index of the values
+// min(prev of the three other edges) over its butterflies whose three other
+// edges are all non-zero; edges with prev[e] == 0 stay 0; rounds repeat until
+// one changes nothing (that round is counted).
+//
+// With the bloom index a butterfly is a pair of middles (i, j) of one bloom.
+// For edge e in middle i (partner edge e' of the same middle) the values are
+// min(prev[e'], mv[j]) for every other middle j of the bloom, where
+// mv[j] = min(prev of j's two edges) is computed once per round. A zero
+// value is a butterfly the reference skips; it never counts towards an
+// h-index >= 1, so it is simply not binned.
+//
+// h-index of one edge's values: it never exceeds prev[e] (the rounds are
+// non-increasing), so values are clipped to prev[e] and binned into kHB
+// bins of an LDS histogram; when the answer's bin is wider than one value
+// the values are streamed again inside that bin (two passes up to
+// prev[e] ~ 10^6, a third above). One wave per edge; the wave walks the
+// concatenation of the edge's blooms 64 middles at a time.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "bt_common.h"
+#include "bt_internal.h"
+
+using namespace bt;
+
+namespace {
+
+constexpr int kHT = 512;              // block size
+constexpr int kHW = kHT / kWave;      // waves per block
+constexpr uint32_t kHB = 1024;        // histogram bins per wave
+
+__global__ void k_hidx_mv(uint32_t R, const uint64_t* __restrict__ rec, const uint64_t* __restrict__ prev,
+                          uint64_t* __restrict__ mv) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < R; p += gridDim.x * blockDim.x) {
+        uint64_t const r = rec[p];
+        uint64_t const a = prev[uint32_t(r >> 32)], b = prev[uint32_t(r)];
+        mv[p] = a < b ? a : b;
+    }
+}
+
+struct HidxArgs {
+    uint32_t m;
+    const uint64_t* prev;
+    uint64_t* curr;
+    const uint32_t* inc_off;
+    const uint32_t* inc;     // per incidence: record (middle) holding the edge
+    const uint64_t* rec;     // middles: id(r-v) << 32 | id(v-w)
+    const uint32_t* bid;     // per record: bloom index + 1
+    const uint32_t* bstart;  // per bloom: first record, then R
+    const uint64_t* mv;      // per record: min(prev) of its two edges
+    unsigned long long* changed;
+};
+
+// Streams the values of edge e's butterflies (one wave), clips them to cap
+// (= prev[e], which bounds the answer) and bins those in [lo, hi] into
+// hist[(v - lo) / width]; values above hi are the ones already counted
+// above the range.
+__device__ __forceinline__ void bin_values(const HidxArgs& a, uint32_t e, uint32_t q0, uint32_t q1, uint64_t lo,
+                                           uint64_t hi, uint64_t width, uint64_t cap, uint32_t* hist, uint32_t* s_incl,
+                                           uint32_t* s_base, uint32_t* s_skip, uint64_t* s_pp) {
+    int const lane = threadIdx.x & (kWave - 1);
+    for (uint32_t b0 = q0; b0 < q1; b0 += kWave) {
+        uint32_t const q = b0 + lane;
+        uint32_t len = 0, base = 0, skip = 0;
+        uint64_t pp = 0;
+        if (q < q1) {
+            uint32_t const p = a.inc[q];
+            uint64_t const r = a.rec[p];
+            uint32_t const partner = uint32_t(r >> 32) == e ? uint32_t(r) : uint32_t(r >> 32);
+            pp = a.prev[partner];
+            uint32_t const b = a.bid[p] - 1;
+            base = a.bstart[b];
+            skip = p;
+            // a zero partner makes every value of this bloom zero
+            len = pp ? a.bstart[b + 1] - base : 0u;
+        }
+        uint32_t const incl = wave_incl_scan(len);
+        uint32_t const total = __shfl(incl, kWave - 1, kWave);
+        __builtin_amdgcn_wave_barrier();
+        s_incl[lane] = incl;
+        s_base[lane] = base - (incl - len);  // record index minus flattened position
+        s_skip[lane] = skip;
+        s_pp[lane] = pp;
+        __builtin_amdgcn_wave_barrier();
+        uint32_t const nb = b0 + kWave <= q1 ? uint32_t(kWave) : q1 - b0;
+        for (uint32_t t0 = 0; t0 < total; t0 += kWave) {
+            uint32_t const t = t0 + lane;
+            if (t < total) {
+                // the bloom of flattened position t: first prefix above t
+                uint32_t j = 0;
+#pragma unroll
+                for (uint32_t step = kWave / 2; step > 0; step >>= 1) {
+                    uint32_t const k = j + step;
+                    j = (k <= nb && s_incl[k - 1] <= t) ? k : j;
+                }
+                uint32_t const idx = s_base[j] + t;
+                if (idx != s_skip[j]) {
+                    uint64_t v = a.mv[idx];
+                    uint64_t const pj = s_pp[j];
+                    v = pj < v ? pj : v;
+                    v = v < cap ? v : cap;
+                    if (v != 0 && v >= lo && v <= hi) atomicAdd(&hist[(v - lo) / width], 1u);
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+__global__ __launch_bounds__(kHT) void k_hidx_round(HidxArgs a) {
+    __shared__ uint32_t s_hist[kHW][kHB];
+    __shared__ uint32_t s_incl[kHW][kWave], s_base[kHW][kWave], s_skip[kHW][kWave];
+    __shared__ uint64_t s_pp[kHW][kWave];
+    int const lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    uint32_t* hist = s_hist[wv];
+    uint32_t const nw = gridDim.x * kHW;
+    bool changed = false;
+    for (uint32_t e = blockIdx.x * kHW + wv; e < a.m; e += nw) {
+        uint64_t const pe = a.prev[e];
+        if (pe == 0) {  // cpu_bitruss.cpp:128
+            if (lane == 0) a.curr[e] = 0;
+            continue;
+        }
+        uint32_t const q0 = a.inc_off[e], q1 = a.inc_off[e + 1];
+        // candidates [lo, hi]; `above` values lie above hi
+        uint64_t lo = 0, hi = pe, above = 0;
+        uint64_t width = (pe + kHB) / kHB;  // ceil((pe + 1) / kHB)
+        uint64_t h = 0;
+        for (;;) {
+            for (uint32_t i = lane; i < kHB; i += kWave) hist[i] = 0;
+            __builtin_amdgcn_wave_barrier();
+            bin_values(a, e, q0, q1, lo, hi, width, pe, hist, s_incl[wv], s_base[wv], s_skip[wv], s_pp[wv]);
+            __builtin_amdgcn_wave_barrier();
+            // largest bin b whose lower end x_b = lo + b * width has
+            // #{values >= x_b} >= x_b; lane L holds bins [L * per, (L+1) * per)
+            uint32_t const nbins = uint32_t((hi - lo) / width + 1);
+            uint32_t const per = (nbins + kWave - 1) / kWave;
+            uint32_t const b_lo = lane * per, b_hi = std::min(nbins, b_lo + per);
+            uint64_t mine = 0;
+            for (uint32_t i = b_lo; i < b_hi; i++) mine += hist[i];
+            // values in the bins of higher lanes
+            uint64_t suf = mine;
+#pragma unroll
+            for (int d = 1; d < kWave; d <<= 1) {
+                uint64_t const o = __shfl_down(suf, d, kWave);
+                if (lane + d < kWave) suf += o;
+            }
+            uint64_t acc = above + (suf - mine);
+            int found = -1;
+            uint64_t found_above = 0;
+            for (uint32_t i = b_hi; i-- > b_lo;) {
+                uint64_t const above_i = acc;
+                acc += hist[i];
+                if (acc >= lo + uint64_t(i) * width) {
+                    found = int(i);
+                    found_above = above_i;
+                    break;
+                }
+            }
+            // the highest lane with a hit holds the answer's bin (bin 0 always
+            // qualifies: its lower end lo has #{values >= lo} >= lo)
+            uint64_t const hits = __ballot(found >= 0);
+            int const src = hits ? 63 - __builtin_clzll(hits) : 0;
+            uint32_t const b = uint32_t(__shfl(found, src, kWave));
+            uint64_t const ab = __shfl(found_above, src, kWave);
+            uint64_t const x = lo + uint64_t(b) * width;
+            __builtin_amdgcn_wave_barrier();
+            if (width == 1) {
+                h = x;
+                break;
+            }
+            above = ab;
+            lo = x;
+            hi = std::min(x + width - 1, hi);
+            width = (hi - lo + kHB) / kHB;  // ceil((hi - lo + 1) / kHB)
+        }
+        if (lane == 0) a.curr[e] = h;
+        changed |= h != pe;
+    }
+    if (changed && lane == 0) __hip_atomic_store(a.changed, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace
+
+int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds_out) {
+    bt_ctx* c = dg->ctx;
+    hipStream_t s = c->stream;
+    uint32_t const m = dg->m;
+    bt_timings const count_t = c->last;
+    bt_timer tm(c);
+    BT_TRY(tm.start(5));
+    BT_TRY(bt_index_build(dg, true));
+    bt_index& ix = c->ix;
+    uint32_t const R = uint32_t(ix.R);
+    uint32_t rounds = 0;
+    if (m) {
+        // mv reuses the unsorted record keys (R u64, free after the sort)
+        BT_TRY(bt_buf_reserve(ix.keys0, sizeof(uint64_t) * std::max<uint32_t>(R, 1)));
+        uint64_t* mv = static_cast<uint64_t*>(ix.keys0.ptr);
+        BT_HIP_TRY(hipMemcpyAsync(dg->res, dg->sup, sizeof(uint64_t) * m, hipMemcpyDeviceToDevice, s));
+        uint64_t* prev = dg->res;
+        uint64_t* curr = dg->aux;
+        unsigned long long* changed = reinterpret_cast<unsigned long long*>(dg->ctl + 16);
+        unsigned const grid_mv = grid_for(std::max<uint32_t>(R, 1), 256, unsigned(c->cu_count) * 16);
+        unsigned const grid_e = std::max(1u, std::min(unsigned(c->cu_count) * 2, (m + kHW - 1) / kHW));
+        uint64_t* h_changed = static_cast<uint64_t*>(c->h_pinned);
+        for (;;) {
+            rounds++;
+            BT_HIP_TRY(hipMemsetAsync(changed, 0, sizeof(uint64_t), s));
+            if (R) k_hidx_mv<<<grid_mv, 256, 0, s>>>(R, static_cast<const uint64_t*>(ix.vals1.ptr), prev, mv);
+            HidxArgs a{m,
+                       prev,
+                       curr,
+                       static_cast<const uint32_t*>(ix.inc_off.ptr),
+                       static_cast<const uint32_t*>(ix.inc.ptr),
+                       static_cast<const uint64_t*>(ix.vals1.ptr),
+                       static_cast<const uint32_t*>(ix.bid.ptr),
+                       static_cast<const uint32_t*>(ix.bstart.ptr),
+                       mv,
+                       changed};
+            k_hidx_round<<<grid_e, kHT, 0, s>>>(a);
+            BT_HIP_TRY(hipGetLastError());
+            BT_HIP_TRY(hipMemcpyAsync(h_changed, changed, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+            BT_HIP_TRY(hipStreamSynchronize(s));
+            std::swap(prev, curr);
+            if (!*h_changed) break;
+        }
+        if (prev != dg->res)
+            BT_HIP_TRY(hipMemcpyAsync(dg->res, prev, sizeof(uint64_t) * m, hipMemcpyDeviceToDevice, s));
+    }
+    BT_TRY(tm.start(6));
+    BT_HIP_TRY(hipStreamSynchronize(s));
+    c->last = count_t;
+    c->last.peel_ms = tm.ms(5, 6);
+    c->last.total_ms = count_t.total_ms + c->last.peel_ms;
+    c->last.sub_rounds = rounds;
+    if (rounds_out) *rounds_out = rounds;
+    return BT_OK;
+}

@@ -17,7 +17,8 @@
 //   bclaim[R + ...]  at s: claim stamp of the peel (0 here); then the
 //                    peel's dummy stamps (kClaimDummiesPerBlock per block)
 //   inc_off[m+1]     per-edge offsets into inc
-//   inc[2R]          first records of the blooms holding the edge
+//   inc[2R]          first records of the blooms holding the edge (or, for
+//                    the h-index rounds, the records holding it)
 #include <hip/hip_runtime.h>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -66,10 +67,10 @@ __global__ void k_inc_count(uint64_t R, const uint64_t* __restrict__ rec, uint32
 
 __global__ void k_inc_fill(uint64_t R, const uint64_t* __restrict__ rec, const uint32_t* __restrict__ bid,
                            const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ inc_off,
-                           uint32_t* __restrict__ cur, uint32_t* __restrict__ inc) {
+                           uint32_t* __restrict__ cur, uint32_t* __restrict__ inc, bool records) {
     for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < R; i += uint64_t(gridDim.x) * blockDim.x) {
         uint64_t const v = rec[i];
-        uint32_t const b = bstart[bid[i] - 1];
+        uint32_t const b = records ? uint32_t(i) : bstart[bid[i] - 1];
         uint32_t const e1 = uint32_t(v >> 32), e2 = uint32_t(v);
         inc[inc_off[e1] + atomicAdd(&cur[e1], 1u)] = b;
         inc[inc_off[e2] + atomicAdd(&cur[e2], 1u)] = b;
@@ -84,7 +85,7 @@ int bits_of(uint64_t x) {
 
 }  // namespace
 
-int bt_index_build(bt_dgraph* dg) {
+int bt_index_build(bt_dgraph* dg, bool inc_records) {
     bt_ctx* c = dg->ctx;
     bt_index& ix = c->ix;
     hipStream_t s = c->stream;
@@ -147,7 +148,7 @@ int bt_index_build(bt_dgraph* dg) {
     BT_HIP_TRY(rocprim::exclusive_scan(c->sort_tmp.ptr, t3, cur, inc_off, 0u, size_t(m) + 1,
                                        rocprim::plus<uint32_t>(), s));
     BT_HIP_TRY(hipMemsetAsync(cur, 0, sizeof(uint32_t) * (size_t(m) + 1), s));
-    k_inc_fill<<<grid, T, 0, s>>>(R, v1, bid, bstart, inc_off, cur, inc);
+    k_inc_fill<<<grid, T, 0, s>>>(R, v1, bid, bstart, inc_off, cur, inc, inc_records);
     BT_HIP_TRY(hipGetLastError());
     return BT_OK;
 }

@@ -120,13 +120,16 @@ struct bt_timer {
 // the C-ABI layer).
 int bt_prep_priority(bt_dgraph* dg);
 int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total, uint64_t* support_max, bool emit = false);
-int bt_index_build(bt_dgraph* dg);
+// inc_records: incidences name the record holding the edge instead of the
+// bloom's first record (the h-index rounds need the edge's middle)
+int bt_index_build(bt_dgraph* dg, bool inc_records = false);
 int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels);
 int bt_msp_merge_peel(bt_dgraph* dg, uint32_t* levels);
 // ix.R value meaning "records were not emitted" (index path unavailable)
 constexpr uint64_t kNoIndex = ~0ull;
 int bt_msp_impl(bt_dgraph* dg, uint32_t* levels);
 int bt_hidx_impl(bt_dgraph* dg, uint32_t* rounds);
+int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds);
 
 // Read back a few control words (synchronises the stream).
 int bt_read_ctl(bt_dgraph* dg, int first, int count, uint64_t* out);

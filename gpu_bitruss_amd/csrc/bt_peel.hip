@@ -939,7 +939,6 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     BT_TRY(tm.start(5));
     dg->has_support = false;
     BT_TRY(bt_index_build(dg));
-    BT_TRY(tm.start(6));
     bt_index& ix = c->ix;
     // one block per CU keeps the grid barrier short; block 0 watches every
     // other block with one thread
@@ -995,6 +994,8 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     }
     k_msp_init<<<grid_for(n_arrive, 256, 1u << 16), 256, 0, s>>>(ctl, a.arrive, uint32_t(n_arrive));
     BT_HIP_TRY(hipGetLastError());
+    // events 6..7 bracket the persistent kernel alone
+    BT_TRY(tm.start(6));
     void* params[] = {&a};
     BT_HIP_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_msp_persist), dim3(G), dim3(kMB), params,
                                           0, s));

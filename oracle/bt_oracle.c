@@ -325,9 +325,13 @@ static inline void attribute(peel_ctx* c, uint32_t uv, uint32_t wv, uint32_t wx,
     }
 }
 
-int or_msp(const or_graph* g, int threads, uint64_t* bitruss, uint32_t* levels_out,
-           or_stats* stats) {
+/* max_seconds > 0 stops after the first level that ends past that time
+ * (a bounded sample of the decomposition, for timing); *complete says
+ * whether every edge was peeled. */
+static int msp_body(const or_graph* g, int threads, uint64_t* bitruss, uint32_t* levels_out,
+                    or_stats* stats, double max_seconds, int* complete) {
     if (check_graph(g) || (g->m && !bitruss)) return OR_EINVAL;
+    const double t_start = omp_get_wtime();
     const uint32_t m = g->m, n = g->n;
     uint64_t* sup = bitruss; /* the result is the final support (:536-597) */
     int st = or_ebfc(g, threads, sup, NULL, NULL, stats);
@@ -449,7 +453,9 @@ int or_msp(const or_graph* g, int threads, uint64_t* bitruss, uint32_t* levels_o
             mcur = kept / 2;
         }
         level++;
+        if (max_seconds > 0 && omp_get_wtime() - t_start > max_seconds) break;
     }
+    if (complete) *complete = edge_num <= 0;
     free(processed); free(in_curr); free(in_next); free(curr); free(next);
     free(nbr); free(eid); free(deg);
     if (levels_out) *levels_out = (uint32_t)(level - 1);
@@ -462,6 +468,16 @@ int or_msp(const or_graph* g, int threads, uint64_t* bitruss, uint32_t* levels_o
         stats->sub_rounds += rounds;
     }
     return OR_OK;
+}
+
+int or_msp(const or_graph* g, int threads, uint64_t* bitruss, uint32_t* levels_out,
+           or_stats* stats) {
+    return msp_body(g, threads, bitruss, levels_out, stats, 0.0, NULL);
+}
+
+int or_msp_sample(const or_graph* g, int threads, double max_seconds, uint64_t* work,
+                  uint32_t* levels_done, int* complete, or_stats* stats) {
+    return msp_body(g, threads, work, levels_done, stats, max_seconds, complete);
 }
 
 /* ------------------------------------------------------- textbook peel */

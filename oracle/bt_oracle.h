@@ -52,6 +52,12 @@ int or_hidx(const or_graph* g, int threads, uint64_t* bitruss, uint32_t* rounds,
             or_stats* stats);
 int or_msp(const or_graph* g, int threads, uint64_t* bitruss, uint32_t* levels,
            or_stats* stats);
+/* The msp computation cut short for timing: support counting, then peel
+ * levels until the first level that ends past max_seconds. `work` (m)
+ * holds the supports as they stand; *complete is 1 when every edge got
+ * its result. The stats count the work done. */
+int or_msp_sample(const or_graph* g, int threads, double max_seconds, uint64_t* work,
+                  uint32_t* levels_done, int* complete, or_stats* stats);
 int or_peel_seq(const or_graph* g, uint64_t* bitruss);
 
 #ifdef __cplusplus

@@ -55,8 +55,10 @@ def lib() -> C.CDLL:
         L.or_bfc.argtypes = [C.POINTER(OrGraph), C.c_int, u64p, C.POINTER(OrStats)]
         L.or_hidx.argtypes = [C.POINTER(OrGraph), C.c_int, u64p, C.POINTER(C.c_uint32), C.POINTER(OrStats)]
         L.or_msp.argtypes = [C.POINTER(OrGraph), C.c_int, u64p, C.POINTER(C.c_uint32), C.POINTER(OrStats)]
+        L.or_msp_sample.argtypes = [C.POINTER(OrGraph), C.c_int, C.c_double, u64p, C.POINTER(C.c_uint32),
+                                    C.POINTER(C.c_int), C.POINTER(OrStats)]
         L.or_peel_seq.argtypes = [C.POINTER(OrGraph), u64p]
-        for f in (L.or_ebfc, L.or_bfc, L.or_hidx, L.or_msp, L.or_peel_seq):
+        for f in (L.or_ebfc, L.or_bfc, L.or_hidx, L.or_msp, L.or_msp_sample, L.or_peel_seq):
             f.restype = C.c_int
         _lib = L
     return _lib
@@ -111,6 +113,18 @@ def msp(g, threads: int = 1):
     levels, st = C.c_uint32(), OrStats()
     _run("msp", lib().or_msp(C.byref(v.s), threads, out.ctypes.data_as(u64p), C.byref(levels), C.byref(st)))
     return out[: g.m], levels.value, st.as_dict()
+
+
+def msp_sample(g, threads: int, max_seconds: float):
+    """msp cut short for timing: support counting, then peel levels until
+    the first one that ends past max_seconds. Returns (levels_done,
+    complete, stats) with the work counters of what was done."""
+    v = _View(g)
+    work = np.zeros(max(g.m, 1), np.uint64)
+    levels, done, st = C.c_uint32(), C.c_int(), OrStats()
+    _run("msp_sample", lib().or_msp_sample(C.byref(v.s), threads, max_seconds, work.ctypes.data_as(u64p),
+                                           C.byref(levels), C.byref(done), C.byref(st)))
+    return levels.value, bool(done.value), st.as_dict()
 
 
 def peel_seq(g):

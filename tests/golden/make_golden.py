@@ -6,8 +6,16 @@ without rerunning the slow CPU algorithms.
 
 Each entry holds sha256 of the little-endian uint64 arrays (ebfc supports,
 msp results, hidx results), plus the totals and the oracle work counters.
-The hidx oracle is slow on powerlaw_2m (Jacobi rounds on 8 threads take
-tens of minutes); pass --skip-hidx to leave it out.
+The hidx oracle is slow on powerlaw_2m (its Jacobi rounds took about two
+hours on 5 threads); pass --skip-hidx to leave it out, and merge a run of
+the oracle CLI made separately with --hidx-from:
+
+    oracle/_build/bt_oracle_cli --graph powerlaw_2m.bin --algo hidx \
+        --threads 5 --dump c2_hidx.u64 > c2_hidx.json
+    python tests/golden/make_golden.py powerlaw_2m --skip-hidx \
+        --hidx-from c2_hidx.json c2_hidx.u64
+
+(the .bin is the generated graph written with Graph.graph_to_bin).
 """
 from __future__ import annotations
 
@@ -37,7 +45,11 @@ def main() -> None:
     ap.add_argument("workloads", nargs="+")
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 4)
     ap.add_argument("--skip-hidx", action="store_true")
+    ap.add_argument("--hidx-from", nargs=2, metavar=("JSON", "U64"),
+                    help="oracle CLI output (JSON line, --dump array) of hidx on the one workload")
     args = ap.parse_args()
+    if args.hidx_from and len(args.workloads) != 1:
+        ap.error("--hidx-from takes one workload")
     for wl in args.workloads:
         g = gb.Graph.generate(wl)
         sup, total, smax, st = oracle.ebfc(g, args.threads)
@@ -53,6 +65,14 @@ def main() -> None:
             h, rounds, st3 = oracle.hidx(g, args.threads)
             out.update({"hidx_sha256": digest(h), "hidx_kmax": int(h.max()) if g.m else 0,
                         "hidx_rounds_jacobi": rounds, "hidx_stats": st3})
+        if args.hidx_from:
+            with open(args.hidx_from[0]) as f:
+                run = json.loads(f.read().strip().splitlines()[-1])
+            h = np.fromfile(args.hidx_from[1], dtype="<u8")
+            assert run["algo"] == "hidx" and run["m"] == g.m and h.size == g.m, "hidx run of another graph"
+            out.update({"hidx_sha256": digest(h), "hidx_kmax": int(h.max()) if g.m else 0,
+                        "hidx_rounds_jacobi": run["iters"],
+                        "hidx_stats": {k: run[k] for k in ("w_h", "i_h", "b_h")}})
         path = os.path.join(HERE, f"{wl}.json")
         with open(path, "w") as f:
             json.dump(out, f, indent=1, sort_keys=True)

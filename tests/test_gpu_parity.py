@@ -175,8 +175,19 @@ def test_powerlaw_2m_golden(dev):
     say("powerlaw_2m msp")
     res, levels = dg.msp()
     assert digest(res) == gold["msp_sha256"] and levels == gold["msp_levels"]
-    if "hidx_sha256" in gold:
-        say("powerlaw_2m hidx")
-        h, _ = dg.hidx()
-        assert digest(h) == gold["hidx_sha256"]
+    dg.free()
+
+
+@pytest.mark.gpu
+def test_powerlaw_2m_golden_hidx(dev):
+    """h-index bitruss numbers and Jacobi round count of the 2M-edge graph
+    against the oracle's (its CPU run took about two hours)."""
+    gold = json.load(open(os.path.join(GOLDEN, "powerlaw_2m.json")))
+    g = gb.Graph.generate("powerlaw_2m")
+    dg = dev.upload(g)
+    say("powerlaw_2m hidx")
+    h, rounds = dg.hidx()
+    assert digest(h) == gold["hidx_sha256"]
+    assert int(h.max()) == gold["hidx_kmax"]
+    assert rounds == gold["hidx_rounds_jacobi"]
     dg.free()
