@@ -34,7 +34,7 @@ def _copy(ptr, count: int) -> np.ndarray:
 class Graph:
     """Bipartite CSR graph with the reference's public members.
 
-    ``Graph(filename, is_to_bin)`` follows graph.cpp:315-331: with
+    ``Graph(filename, is_to_bin)`` follows graph.cpp:261-277: with
     ``is_to_bin`` true the file is a text edge list to convert, otherwise it
     must be a ``.bin`` file (the name must contain ".bin").
     """

@@ -76,9 +76,10 @@ int bt_ctx_last_timings(const bt_ctx* ctx, bt_timings* out);
 
 /* Context options (no counterpart in the reference, whose launch shapes are
  * compile-time macros in src/util/config.h:6-13).
- *   BT_OPT_MSP_PATH        BT_MSP_PATH_AUTO (default): bloom-index peel in one
- *                          persistent launch when the index fits, else the
- *                          merge peel; _MERGE / _INDEX force one path
+ *   BT_OPT_MSP_PATH        BT_MSP_PATH_AUTO (default): msp and hidx run over the
+ *                          bloom index when it fits (msp: one persistent
+ *                          launch), else over the merge enumeration of the
+ *                          reference's wedges; _MERGE / _INDEX force one path
  *                          (_INDEX fails with BT_ERR_BUDGET when it cannot run)
  *   BT_OPT_INDEX_MEM_PCT   share of free device memory the index may use
  *                          (1..95, default 50) */
