@@ -5,6 +5,7 @@ peeling (msp), as hand-written HIP kernels for gfx950 behind a C-ABI
 """
 from .graph import Graph, WORKLOADS
 from .device import (
+    Batch,
     Device,
     DeviceGraph,
     bfc_evpp,
@@ -17,6 +18,6 @@ from .device import (
 from ._lib import BitrussError, LIB_PATH, COH_PATH
 
 __all__ = [
-    "Graph", "WORKLOADS", "Device", "DeviceGraph", "bfc_vpp", "bfc_evpp", "g_bitruss_hindex",
+    "Graph", "WORKLOADS", "Batch", "Device", "DeviceGraph", "bfc_vpp", "bfc_evpp", "g_bitruss_hindex",
     "bitruss_msp", "default_device", "device_count", "BitrussError", "LIB_PATH", "COH_PATH",
 ]

@@ -61,6 +61,17 @@ class BtTimings(C.Structure):
     ]
 
 
+class BtBatchResult(C.Structure):
+    """struct bt_batch_result (include/bitruss_hip.h)."""
+
+    _fields_ = [
+        ("total", C.c_uint64),
+        ("support_max", C.c_uint64),
+        ("levels", C.c_uint32),
+        ("rounds", C.c_uint32),
+    ]
+
+
 class BitrussError(RuntimeError):
     """A non-zero status returned by libbitruss_hip.so."""
 
@@ -71,7 +82,11 @@ class BitrussError(RuntimeError):
 
 OPT_MSP_PATH = 1
 OPT_INDEX_MEM_PCT = 2
+OPT_BATCH_WEDGES = 3
+OPT_BATCH_LEVELS = 4
+BATCH_LEVELS = {"per_graph": 0, "union": 1}
 MSP_PATHS = {"auto": 0, "merge": 1, "index": 2}
+ALGOS = {"bfc": 1, "ebfc": 2, "hidx": 3, "msp": 4}
 
 _lib = None
 
@@ -103,6 +118,8 @@ def lib() -> C.CDLL:
              C.c_uint64, C.POINTER(BtEdgeList)],
         ),
         "bt_gen_workload": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(BtEdgeList)]),
+        "bt_graph_wedges": (C.c_int, [C.POINTER(BtHostGraph), C.POINTER(C.c_uint64)]),
+        "bt_graph_union": (C.c_int, [C.POINTER(BtHostGraph), C.c_uint64, C.POINTER(BtHostGraph), u64p]),
     }
     gpu_sig = {
         "bt_device_count": (C.c_int, [C.POINTER(C.c_int)]),
@@ -116,6 +133,10 @@ def lib() -> C.CDLL:
         "bt_ebfc": (C.c_int, [vp, vp, u64p, u64p, u64p]),
         "bt_hidx": (C.c_int, [vp, vp, u64p, C.POINTER(C.c_uint32)]),
         "bt_msp": (C.c_int, [vp, vp, u64p, C.POINTER(C.c_uint32)]),
+        "bt_batch_upload": (C.c_int, [vp, C.POINTER(BtHostGraph), C.c_size_t, C.POINTER(vp)]),
+        "bt_batch_free": (C.c_int, [vp]),
+        "bt_batch_chunks": (C.c_int, [vp, C.POINTER(C.c_size_t)]),
+        "bt_batch_run": (C.c_int, [vp, vp, C.c_int, C.POINTER(u64p), C.POINTER(BtBatchResult)]),
     }
     for name, (res, args) in {**sig, **gpu_sig}.items():
         f = getattr(L, name)

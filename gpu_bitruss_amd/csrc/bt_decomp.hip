@@ -243,8 +243,11 @@ struct HidxShared {
 // the enumeration is repeated inside the bin that holds the answer.
 __global__ __launch_bounds__(kPeelBlock) void k_hidx_round(uint32_t m, Csr g, const uint64_t* __restrict__ prev,
                                                            uint64_t* __restrict__ curr,
-                                                           uint64_t* __restrict__ ctl) {
+                                                           uint64_t* __restrict__ ctl,
+                                                           const uint32_t* __restrict__ egraph,
+                                                           uint32_t* __restrict__ glast, uint32_t round) {
     __shared__ HidxShared sh;
+    uint32_t last_g = ~0u;  // thread 0: graph whose change this block last recorded
     for (uint32_t e = blockIdx.x; e < m; e += gridDim.x) {
         uint64_t const pe = prev[e];
         if (pe == 0) {  // cpu_bitruss.cpp:128
@@ -311,7 +314,13 @@ __global__ __launch_bounds__(kPeelBlock) void k_hidx_round(uint32_t m, Csr g, co
         }
         if (threadIdx.x == 0) {
             curr[e] = sh.h;
-            if (sh.h != pe) ctl[C_CHANGED] = 1;
+            if (sh.h != pe) {
+                ctl[C_CHANGED] = 1;
+                if (egraph && egraph[e] != last_g) {  // batches: per-graph round count
+                    last_g = egraph[e];
+                    __hip_atomic_store(&glast[last_g], round, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
         }
         __syncthreads();
     }
@@ -419,7 +428,9 @@ int bt_hidx_impl(bt_dgraph* dg, uint32_t* rounds_out) {
     bt_timer tm(c);
     BT_TRY(tm.start(5));
     Csr g{dg->off, dg->nbr, dg->eids, dg->edges};
-    uint32_t rounds = 0;
+    // the reference runs one round even on an edgeless graph
+    // (bitruss_hidx.cu:218-231)
+    uint32_t rounds = m ? 0 : 1;
     if (m) {
         BT_HIP_TRY(hipMemcpyAsync(dg->res, dg->sup, sizeof(uint64_t) * m, hipMemcpyDeviceToDevice, s));
         uint64_t* prev = dg->res;
@@ -428,7 +439,7 @@ int bt_hidx_impl(bt_dgraph* dg, uint32_t* rounds_out) {
         while (true) {
             rounds++;
             k_set_u64<<<1, 64, 0, s>>>(dg->ctl + C_CHANGED, 0);
-            k_hidx_round<<<grid, kPeelBlock, 0, s>>>(m, g, prev, curr, dg->ctl);
+            k_hidx_round<<<grid, kPeelBlock, 0, s>>>(m, g, prev, curr, dg->ctl, dg->egraph, dg->graph_last, rounds);
             BT_HIP_TRY(hipGetLastError());
             uint64_t changed = 0;
             BT_TRY(bt_read_ctl(dg, C_CHANGED, 1, &changed));

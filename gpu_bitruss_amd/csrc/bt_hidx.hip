@@ -6,7 +6,8 @@
 // every edge e with prev[e] > 0 takes the h-index of the values
 // min(prev of the three other edges) over its butterflies whose three other
 // edges are all non-zero; edges with prev[e] == 0 stay 0; rounds repeat until
-// one changes nothing (that round is counted).
+// one changes nothing (that round is counted, so an edgeless graph takes one
+// round, bitruss_hidx.cu:218-231).
 //
 // With the bloom index a butterfly is a pair of middles (i, j) of one bloom.
 // For edge e in middle i (partner edge e' of the same middle) the values are
@@ -56,6 +57,9 @@ struct HidxArgs {
     const uint32_t* bstart;  // per bloom: first record, then R
     const uint64_t* mv;      // per record: min(prev) of its two edges
     unsigned long long* changed;
+    const uint32_t* egraph;  // batches: graph of each edge (else null)
+    uint32_t* glast;         // batches: per graph, last round with a change
+    uint32_t round;
 };
 
 // Streams the values of edge e's butterflies (one wave), clips them to cap
@@ -122,6 +126,7 @@ __global__ __launch_bounds__(kHT) void k_hidx_round(HidxArgs a) {
     uint32_t* hist = s_hist[wv];
     uint32_t const nw = gridDim.x * kHW;
     bool changed = false;
+    uint32_t last_g = ~0u;  // graph whose change this wave last recorded
     for (uint32_t e = blockIdx.x * kHW + wv; e < a.m; e += nw) {
         uint64_t const pe = a.prev[e];
         if (pe == 0) {  // cpu_bitruss.cpp:128
@@ -182,7 +187,13 @@ __global__ __launch_bounds__(kHT) void k_hidx_round(HidxArgs a) {
             width = (hi - lo + kHB) / kHB;  // ceil((hi - lo + 1) / kHB)
         }
         if (lane == 0) a.curr[e] = h;
-        changed |= h != pe;
+        if (h != pe) {
+            changed = true;
+            if (a.egraph && a.egraph[e] != last_g) {  // batches: per-graph round count
+                last_g = a.egraph[e];
+                if (lane == 0) __hip_atomic_store(&a.glast[last_g], a.round, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
     }
     if (changed && lane == 0) __hip_atomic_store(a.changed, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -199,7 +210,8 @@ int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds_out) {
     BT_TRY(bt_index_build(dg, true));
     bt_index& ix = c->ix;
     uint32_t const R = uint32_t(ix.R);
-    uint32_t rounds = 0;
+    // the reference runs one round even on an edgeless graph
+    uint32_t rounds = m ? 0 : 1;
     if (m) {
         // mv reuses the unsorted record keys (R u64, free after the sort)
         BT_TRY(bt_buf_reserve(ix.keys0, sizeof(uint64_t) * std::max<uint32_t>(R, 1)));
@@ -224,7 +236,10 @@ int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds_out) {
                        static_cast<const uint32_t*>(ix.bid.ptr),
                        static_cast<const uint32_t*>(ix.bstart.ptr),
                        mv,
-                       changed};
+                       changed,
+                       dg->egraph,
+                       dg->graph_last,
+                       rounds};
             k_hidx_round<<<grid_e, kHT, 0, s>>>(a);
             BT_HIP_TRY(hipGetLastError());
             BT_HIP_TRY(hipMemcpyAsync(h_changed, changed, sizeof(uint64_t), hipMemcpyDeviceToHost, s));

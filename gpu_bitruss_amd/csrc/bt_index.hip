@@ -16,6 +16,7 @@
 //                    (r is an upper vertex) << 31; a bloom is named by s
 //   bclaim[R + ...]  at s: claim stamp of the peel (0 here); then the
 //                    peel's dummy stamps (kClaimDummiesPerBlock per block)
+//   bgraph[R]        batches only, at s: the graph of the bloom
 //   inc_off[m+1]     per-edge offsets into inc
 //   inc[2R]          first records of the blooms holding the edge (or, for
 //                    the h-index rounds, the records holding it)
@@ -46,7 +47,9 @@ __global__ void k_bloom_start(uint64_t R, const uint64_t* __restrict__ keys, con
 
 __global__ void k_bloom_size(uint32_t nbloom, uint32_t R, int nb, const uint64_t* __restrict__ keys,
                              const uint8_t* __restrict__ r_up, uint32_t* __restrict__ bstart,
-                             uint32_t* __restrict__ bk, uint32_t* __restrict__ bclaim) {
+                             uint32_t* __restrict__ bk, uint32_t* __restrict__ bclaim,
+                             const uint64_t* __restrict__ rec, const uint32_t* __restrict__ egraph,
+                             uint32_t* __restrict__ bgraph) {
     uint32_t const b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b == 0) bstart[nbloom] = R;
     if (b >= nbloom) return;
@@ -55,6 +58,7 @@ __global__ void k_bloom_size(uint32_t nbloom, uint32_t R, int nb, const uint64_t
     uint32_t const r = uint32_t(keys[s] >> nb);
     bk[s] = (e - s) | (uint32_t(r_up[r]) << 31);
     bclaim[s] = 0;
+    if (egraph) bgraph[s] = egraph[uint32_t(rec[s] >> 32)];  // batches: the bloom's graph
 }
 
 __global__ void k_inc_count(uint64_t R, const uint64_t* __restrict__ rec, uint32_t* __restrict__ cnt) {
@@ -141,7 +145,13 @@ int bt_index_build(bt_dgraph* dg, bool inc_records) {
     auto* bclaim = static_cast<uint32_t*>(ix.bclaim.ptr);
     auto* inc = static_cast<uint32_t*>(ix.inc.ptr);
     k_bloom_start<<<grid, T, 0, s>>>(R, k1, bid, bstart);
-    k_bloom_size<<<grid_for(nbloom, T, 1u << 30), T, 0, s>>>(nbloom, uint32_t(R), nb, k1, dg->r_up, bstart, bk, bclaim);
+    uint32_t* bgraph = nullptr;
+    if (dg->egraph) {
+        BT_TRY(bt_buf_reserve(ix.bgraph, sizeof(uint32_t) * R));
+        bgraph = static_cast<uint32_t*>(ix.bgraph.ptr);
+    }
+    k_bloom_size<<<grid_for(nbloom, T, 1u << 30), T, 0, s>>>(nbloom, uint32_t(R), nb, k1, dg->r_up, bstart, bk, bclaim,
+                                                              v1, dg->egraph, bgraph);
     // edge -> blooms incidence lists
     BT_HIP_TRY(hipMemsetAsync(cur, 0, sizeof(uint32_t) * (size_t(m) + 1), s));
     k_inc_count<<<grid, T, 0, s>>>(R, v1, cur);

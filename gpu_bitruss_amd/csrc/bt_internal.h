@@ -53,6 +53,8 @@ struct bt_index {
     bt_buf inc;           // u32 per incidence: first record of the bloom
     bt_buf touched;       // u32 3 x blooms: shared claimed-bloom lists
     bt_buf area;          // u32 3 x blocks x cap: per-block claimed-bloom lists
+    bt_buf bgraph;        // u32 per record, used at bloom starts: graph of the bloom (batches)
+    bt_buf glev;          // u64 2 x graphs: per-graph levels of the peel (batches)
     bt_buf live;          // u32 per edge: per-block live-edge segments
     bt_buf seg;           // u32 per block + 1: segment offsets, then counts
     uint64_t R = 0, NB = 0;
@@ -74,6 +76,8 @@ struct bt_ctx {
     int coop = 0;                  // cooperative launch supported
     int msp_path = 0;              // BT_MSP_PATH_*
     double index_mem_frac = 0.5;   // share of free device memory the index may use
+    uint64_t batch_wedges = 0;     // wedge bound per batch union (0: the index limits)
+    int batch_levels = 1;          // BT_BATCH_LEVELS_*: how a union's msp walks levels
 };
 
 // Device-resident graph plus the per-graph scratch of the algorithms.
@@ -106,6 +110,11 @@ struct bt_dgraph {
     uint64_t* r_key2 = nullptr; // 2m
     uint32_t* r_val2 = nullptr; // 2m
     bool has_support = false;   // sup holds the ebfc result of this graph
+    // batches (the graph is a disjoint union): graph of each edge, and per
+    // graph the last hidx round that changed one of its edges
+    const uint32_t* egraph = nullptr;  // m
+    uint32_t* graph_last = nullptr;    // graphs of the batch
+    uint32_t ngraph = 0;               // graphs of the batch (0: not a batch)
 };
 
 // per-call helpers

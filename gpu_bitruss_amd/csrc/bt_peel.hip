@@ -73,6 +73,7 @@ constexpr int kSp = 32;                 // u32 spacing of the barrier slots (128
 constexpr uint64_t kInf = ~0ull;
 constexpr uint64_t kBarrierLimit = 30ull * 100000000ull;  // 30 s of the 100 MHz clock
 constexpr unsigned kRelErr = 0x80000000u;                   // release word: error bit
+constexpr uint32_t kSpan = 1024;        // graphs of a block's segment with an LDS minimum
 
 struct MspCtl {
     unsigned rel;                    // barrier release epoch | kRelErr
@@ -113,6 +114,12 @@ struct MspArgs {
     MspCtl* ctl;
     uint64_t* trace;       // optional: per sub-round (claimed, slowest-wave ticks)
     uint32_t trace_cap;
+    // batches (the graph is a disjoint union; each graph peels its own levels)
+    uint32_t ngraph;
+    const uint32_t* egraph;  // m: graph of each edge
+    const uint32_t* bgraph;  // at a bloom's first record: its graph
+    uint64_t* glev;          // [2][ngraph] per-graph level of the current / next step
+    uint32_t* any;           // per block: live edges left
 };
 
 // relaxed device-scope accesses (coherent across XCDs)
@@ -654,7 +661,9 @@ __device__ __forceinline__ void process_large(const MspArgs& a, uint32_t s, uint
     }
 }
 
+template <bool kBatch>
 __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
+    __shared__ uint64_t s_gmin[kBatch ? kSpan : 1];
     __shared__ uint32_t s_part[kWPB + 1];
     __shared__ uint64_t s_red[kWPB + 1];
     __shared__ uint64_t s_c2[kWPB][kC2Cap];
@@ -750,153 +759,299 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
         set_slots(1, block_min_all(mn, s_red), kInf);
     }
     if (!grid_sync(a, G, ++epoch, &s_ok)) return;
-    uint64_t L = level_min(1);
     uint32_t X = 2, k = 0;
     uint32_t nt = 0;            // claimed blooms of sub-round X
     uint64_t t_col = 0, t_pe = 0, w_col = 0, w_pe = 0, pmax_sum = 0;
-    while (L != kInf) {
-        uint32_t const p = k & 1;
-        // ---- collect: frontier of level L from this block's live segment
-        {
-            uint64_t const tp0 = wall_clock64();
-            if (B == 0 && tid == 0) ast(&ctl->ovf_n[(X + 1) % 3], 0u);
-            Out const out = out_for(X);
-            uint32_t const cnt = a.live_cnt[B];
-            uint32_t w = seg0;
-            uint64_t mn = kInf;
-            CrossBuf xb{s_xl[wv], 0, ws};
-            constexpr int U = 4;
-            for (uint32_t base = seg0; base < seg0 + cnt; base += kMB * U) {
-                xb.ensure(U * kWave, X, a, out);
-                // an edge finished at an earlier level has support below L
-                // (it was peeled at a level < L with support <= that level);
-                // a live edge has support >= L
-                uint32_t e[U];
-                uint64_t v[U];
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    uint32_t const i = base + u * kMB + tid;
-                    e[u] = i < seg0 + cnt ? a.live[i] : 0u;
-                }
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    uint32_t const i = base + u * kMB + tid;
-                    v[u] = i < seg0 + cnt ? ald(&a.sup[e[u]]) : 0ull;
-                }
-                uint32_t nk = 0;
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    uint32_t const i = base + u * kMB + tid;
-                    bool const in = i < seg0 + cnt;
-                    nk += (in && v[u] > L) ? 1u : 0u;
-                    if (in && v[u] > L) mn = v[u] < mn ? v[u] : mn;
-                }
-                uint32_t tot = 0;
-                uint32_t pos = w + block_excl_scan<kMB>(nk, s_part, tot);
-                w += tot;
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    uint32_t const i = base + u * kMB + tid;
-                    bool const in = i < seg0 + cnt;
-                    if (in && v[u] > L) a.live[pos++] = e[u];
-                    bool const take = in && v[u] == L;
-                    if (take) {
-                        ast(&a.stamp[e[u]], X);
-                        a.res[e[u]] = L;
-                    }
-                    xb.put(take, e[u]);
-                }
-            }
-            if (tid == 0) a.live_cnt[B] = w - seg0;
-            set_slots(p, block_min_all(mn, s_red), kInf);
-            if ((tid & (kWave - 1)) == 0) s_xn[wv] = xb.n;
-            __syncthreads();
-            claim_block(a, X, out, s_xl, s_xn, s_incl, s_off, s_part);
-            close_out(X);
-            uint64_t const tp1 = wall_clock64();
-            if (!grid_sync(a, G, ++epoch, &s_ok)) return;
-            nt = open_list(X);
-            uint64_t const tp2 = wall_clock64();
-            t_col += tp2 - tp0;
-            w_col += tp1 - tp0;
+
+    // ---- one peel sub-round X over the claimed blooms; level_of(bg) is the
+    // level of the bloom's graph (bg = its graph in a batch, else 0); the
+    // number of blooms claimed for X + 1 is returned to every thread
+    auto peel_sub_round = [&](auto level_of, uint64_t& tmin, uint64_t& dec_written, uint32_t p) -> uint32_t {
+        uint64_t const tp0 = wall_clock64();
+        if (B == 0 && tid == 0) ast(&ctl->ovf_n[(X + 2) % 3], 0u);
+        Out const out = out_for(X + 1);
+        Dec dec{a.sup, a.stamp, a.res, 0, X, tmin};
+        uint64_t* c2 = s_c2[wv];
+        CrossBuf xb{s_xl[wv], 0, ws};
+        int const lane = tid & (kWave - 1);
+        uint64_t const tw0 = wall_clock64();
+        // software pipeline over this wave's blooms: the list entry two
+        // blooms ahead and the header and first 64 middles of the next one
+        // load while the current bloom is processed
+        uint32_t s_next = 0, s_next2 = 0, hdr_next = 0, bg_next = 0;
+        uint64_t r0_next = 0;
+        if (gw < nt) {
+            s_next = list_entry(X, gw);
+            hdr_next = ald(&a.hdr[s_next]);
+            r0_next = (s_next + lane < a.R) ? ald(&a.rec[s_next + lane]) : 0;
+            if constexpr (kBatch) bg_next = a.bgraph[s_next];
         }
-        // ---- peel sub-rounds X, X + 1, ... until one claims nothing
-        uint64_t tmin = kInf;   // this thread's smallest decremented support of the level
-        uint64_t dec_written = kInf;
-        for (;;) {
-            uint64_t const tp0 = wall_clock64();
-            if (B == 0 && tid == 0) ast(&ctl->ovf_n[(X + 2) % 3], 0u);
-            Out const out = out_for(X + 1);
-            Dec dec{a.sup, a.stamp, a.res, L, X, tmin};
-            uint64_t* c2 = s_c2[wv];
-            CrossBuf xb{s_xl[wv], 0, ws};
-            int const lane = tid & (kWave - 1);
-            uint64_t const tw0 = wall_clock64();
-            // software pipeline over this wave's blooms: the list entry two
-            // blooms ahead and the header and first 64 middles of the next
-            // one load while the current bloom is processed
-            uint32_t s_next = 0, s_next2 = 0, hdr_next = 0;
-            uint64_t r0_next = 0;
-            if (gw < nt) {
-                s_next = list_entry(X, gw);
+        if (gw + nw < nt) s_next2 = list_entry(X, gw + nw);
+        for (uint32_t ti = gw; ti < nt; ti += nw) {
+            uint32_t const s = s_next, hdr = hdr_next, bg = bg_next;
+            uint64_t const r0 = r0_next;
+            if (ti + nw < nt) {
+                s_next = s_next2;
                 hdr_next = ald(&a.hdr[s_next]);
                 r0_next = (s_next + lane < a.R) ? ald(&a.rec[s_next + lane]) : 0;
+                if constexpr (kBatch) bg_next = a.bgraph[s_next];
+                if (ti + 2 * nw < nt) s_next2 = list_entry(X, ti + 2 * nw);
             }
-            if (gw + nw < nt) s_next2 = list_entry(X, gw + nw);
-            for (uint32_t ti = gw; ti < nt; ti += nw) {
-                uint32_t const s = s_next, hdr = hdr_next;
-                uint64_t const r0 = r0_next;
-                if (ti + nw < nt) {
-                    s_next = s_next2;
-                    hdr_next = ald(&a.hdr[s_next]);
-                    r0_next = (s_next + lane < a.R) ? ald(&a.rec[s_next + lane]) : 0;
-                    if (ti + 2 * nw < nt) s_next2 = list_entry(X, ti + 2 * nw);
-                }
-                uint32_t const kb = hdr & 0x7FFFFFFFu, up = hdr >> 31;
-                if (kb < 2) continue;
-                // a bloom frees at most two edges per middle
-                xb.ensure(2 * (kb < uint32_t(16 * kWave) ? kb : uint32_t(16 * kWave)), X + 1, a, out);
-                if (kb <= uint32_t(kWave)) process_bloom<1>(a, s, kb, up, r0, X, dec, c2, xb, out);
-                else if (kb <= uint32_t(4 * kWave)) process_bloom<4>(a, s, kb, up, r0, X, dec, c2, xb, out);
-                else if (kb <= uint32_t(16 * kWave)) process_bloom<16>(a, s, kb, up, r0, X, dec, c2, xb, out);
-                else process_large(a, s, kb, up, X, dec, c2, xb, out);
-            }
-            tmin = dec.tmin;
-            uint64_t const tw = wall_clock64() - tw0;
-            if (lane == 0) s_xn[wv] = xb.n;
+            uint32_t const kb = hdr & 0x7FFFFFFFu, up = hdr >> 31;
+            if (kb < 2) continue;
+            dec.L = level_of(bg);
+            // a bloom frees at most two edges per middle
+            xb.ensure(2 * (kb < uint32_t(16 * kWave) ? kb : uint32_t(16 * kWave)), X + 1, a, out);
+            if (kb <= uint32_t(kWave)) process_bloom<1>(a, s, kb, up, r0, X, dec, c2, xb, out);
+            else if (kb <= uint32_t(4 * kWave)) process_bloom<4>(a, s, kb, up, r0, X, dec, c2, xb, out);
+            else if (kb <= uint32_t(16 * kWave)) process_bloom<16>(a, s, kb, up, r0, X, dec, c2, xb, out);
+            else process_large(a, s, kb, up, X, dec, c2, xb, out);
+        }
+        tmin = dec.tmin;
+        uint64_t const tw = wall_clock64() - tw0;
+        if (lane == 0) s_xn[wv] = xb.n;
+        if constexpr (!kBatch) {
             // publish this block's smallest decremented support of the level
             uint64_t const bdec = block_min_all(tmin, s_red);
             if (tid == 0 && bdec < dec_written) ast(&a.slots[size_t(p) * 2 * G + G + B], bdec);
             dec_written = bdec < dec_written ? bdec : dec_written;
-            claim_block(a, X + 1, out, s_xl, s_xn, s_incl, s_off, s_part);
-            close_out(X + 1);
-            if (a.diag) {
-                uint64_t const mx = ~block_min_all(~tw, s_red);  // slowest wave of the block
-                if (tid == 0) afetch_max(&ctl->pmax[X & 1], (unsigned long long)mx);
-            }
-            uint64_t const tp1 = wall_clock64();
-            if (!grid_sync(a, G, ++epoch, &s_ok)) return;
-            uint32_t const nn = open_list(X + 1);
-            uint64_t const tp2 = wall_clock64();
-            t_pe += tp2 - tp0;
-            w_pe += tp1 - tp0;
-            if (a.diag && B == 0 && tid == 0) {
-                unsigned long long const pm = ald(&ctl->pmax[X & 1]);
-                ast(&ctl->pmax[X & 1], 0ull);
-                pmax_sum += pm;
-                int lb = 0;
-                while (lb < 23 && (pm >> (lb + 1))) lb++;
-                ctl->hist[lb]++;
-                if (a.trace && X < a.trace_cap) a.trace[X] = (uint64_t(nn) << 32) | (pm & 0xFFFFFFFFull);
-            }
-            if (!nn) break;
-            nt = nn;
-            X++;
+        } else {
+            __syncthreads();
         }
-        // no bloom claimed for X + 1: the level is over (X + 1 is skipped)
-        L = level_min(p);
-        X += 2;
-        k++;
+        claim_block(a, X + 1, out, s_xl, s_xn, s_incl, s_off, s_part);
+        close_out(X + 1);
+        if (a.diag) {
+            uint64_t const mx = ~block_min_all(~tw, s_red);  // slowest wave of the block
+            if (tid == 0) afetch_max(&ctl->pmax[X & 1], (unsigned long long)mx);
+        }
+        uint64_t const tp1 = wall_clock64();
+        if (!grid_sync(a, G, ++epoch, &s_ok)) return ~0u;
+        uint32_t const nn = open_list(X + 1);
+        uint64_t const tp2 = wall_clock64();
+        t_pe += tp2 - tp0;
+        w_pe += tp1 - tp0;
+        if (a.diag && B == 0 && tid == 0) {
+            unsigned long long const pm = ald(&ctl->pmax[X & 1]);
+            ast(&ctl->pmax[X & 1], 0ull);
+            pmax_sum += pm;
+            int lb = 0;
+            while (lb < 23 && (pm >> (lb + 1))) lb++;
+            ctl->hist[lb]++;
+            if (a.trace && X < a.trace_cap) a.trace[X] = (uint64_t(nn) << 32) | (pm & 0xFFFFFFFFull);
+        }
+        return nn;
+    };
+
+    if constexpr (!kBatch) {
+        uint64_t L = level_min(1);
+        while (L != kInf) {
+            uint32_t const p = k & 1;
+            // ---- collect: frontier of level L from this block's live segment
+            {
+                uint64_t const tp0 = wall_clock64();
+                if (B == 0 && tid == 0) ast(&ctl->ovf_n[(X + 1) % 3], 0u);
+                Out const out = out_for(X);
+                uint32_t const cnt = a.live_cnt[B];
+                uint32_t w = seg0;
+                uint64_t mn = kInf;
+                CrossBuf xb{s_xl[wv], 0, ws};
+                constexpr int U = 4;
+                for (uint32_t base = seg0; base < seg0 + cnt; base += kMB * U) {
+                    xb.ensure(U * kWave, X, a, out);
+                    // an edge finished at an earlier level has support below L
+                    // (it was peeled at a level < L with support <= that level);
+                    // a live edge has support >= L
+                    uint32_t e[U];
+                    uint64_t v[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        uint32_t const i = base + u * kMB + tid;
+                        e[u] = i < seg0 + cnt ? a.live[i] : 0u;
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        uint32_t const i = base + u * kMB + tid;
+                        v[u] = i < seg0 + cnt ? ald(&a.sup[e[u]]) : 0ull;
+                    }
+                    uint32_t nk = 0;
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        uint32_t const i = base + u * kMB + tid;
+                        bool const in = i < seg0 + cnt;
+                        nk += (in && v[u] > L) ? 1u : 0u;
+                        if (in && v[u] > L) mn = v[u] < mn ? v[u] : mn;
+                    }
+                    uint32_t tot = 0;
+                    uint32_t pos = w + block_excl_scan<kMB>(nk, s_part, tot);
+                    w += tot;
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        uint32_t const i = base + u * kMB + tid;
+                        bool const in = i < seg0 + cnt;
+                        if (in && v[u] > L) a.live[pos++] = e[u];
+                        bool const take = in && v[u] == L;
+                        if (take) {
+                            ast(&a.stamp[e[u]], X);
+                            a.res[e[u]] = L;
+                        }
+                        xb.put(take, e[u]);
+                    }
+                }
+                if (tid == 0) a.live_cnt[B] = w - seg0;
+                set_slots(p, block_min_all(mn, s_red), kInf);
+                if ((tid & (kWave - 1)) == 0) s_xn[wv] = xb.n;
+                __syncthreads();
+                claim_block(a, X, out, s_xl, s_xn, s_incl, s_off, s_part);
+                close_out(X);
+                uint64_t const tp1 = wall_clock64();
+                if (!grid_sync(a, G, ++epoch, &s_ok)) return;
+                nt = open_list(X);
+                uint64_t const tp2 = wall_clock64();
+                t_col += tp2 - tp0;
+                w_col += tp1 - tp0;
+            }
+            // ---- peel sub-rounds X, X + 1, ... until one claims nothing
+            uint64_t tmin = kInf;  // this thread's smallest decremented support of the level
+            uint64_t dec_written = kInf;
+            for (;;) {
+                uint32_t const nn = peel_sub_round([&](uint32_t) { return L; }, tmin, dec_written, p);
+                if (nn == ~0u) return;
+                if (!nn) break;
+                nt = nn;
+                X++;
+            }
+            // no bloom claimed for X + 1: the level is over (X + 1 is skipped)
+            L = level_min(p);
+            X += 2;
+            k++;
+        }
+    } else {
+        // Batch: level steps. In step t every graph with live edges peels
+        // its own next level L_g = min support of its live edges, so each
+        // graph goes through exactly the levels and sub-rounds of its plain
+        // run while the phases are shared. Liveness is the stamp (0 = live).
+        uint32_t const gA = seg1 > seg0 ? a.egraph[seg0] : 0u;
+        uint32_t const gB = seg1 > seg0 ? a.egraph[seg1 - 1] : 0u;
+        uint32_t const span = seg1 > seg0 ? gB - gA + 1 : 0u;
+        bool const lds_min = span <= kSpan;
+        for (;;) {
+            uint64_t* gl = a.glev + size_t(k & 1) * a.ngraph;
+            uint64_t* gl_next = a.glev + size_t((k + 1) & 1) * a.ngraph;
+            // ---- level: drop finished edges, per-graph min of live supports
+            {
+                uint64_t const tp0 = wall_clock64();
+                for (uint32_t i = tid; i < span; i += kMB) ast(&gl_next[gA + i], kInf);
+                if (lds_min)
+                    for (uint32_t i = tid; i < span; i += kMB) s_gmin[i] = kInf;
+                __syncthreads();
+                uint32_t const cnt = a.live_cnt[B];
+                uint32_t w = seg0;
+                constexpr int U = 4;
+                for (uint32_t base = seg0; base < seg0 + cnt; base += kMB * U) {
+                    uint32_t e[U], st[U], g[U];
+                    uint64_t v[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        uint32_t const i = base + u * kMB + tid;
+                        e[u] = i < seg0 + cnt ? a.live[i] : 0u;
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        uint32_t const i = base + u * kMB + tid;
+                        bool const in = i < seg0 + cnt;
+                        st[u] = in ? ald(&a.stamp[e[u]]) : 1u;
+                        v[u] = in ? ald(&a.sup[e[u]]) : 0ull;
+                        g[u] = in ? a.egraph[e[u]] : 0u;
+                    }
+                    uint32_t nk = 0;
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        if (st[u] != 0) continue;
+                        nk++;
+                        if (lds_min) atomicMin(reinterpret_cast<unsigned long long*>(&s_gmin[g[u] - gA]),
+                                               (unsigned long long)v[u]);
+                        else __hip_atomic_fetch_min(&gl[g[u]], v[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    uint32_t tot = 0;
+                    uint32_t pos = w + block_excl_scan<kMB>(nk, s_part, tot);
+                    w += tot;
+#pragma unroll
+                    for (int u = 0; u < U; u++)
+                        if (st[u] == 0) a.live[pos++] = e[u];
+                }
+                if (tid == 0) {
+                    a.live_cnt[B] = w - seg0;
+                    ast(&a.any[B], w > seg0 ? 1u : 0u);
+                }
+                __syncthreads();
+                if (lds_min)
+                    for (uint32_t i = tid; i < span; i += kMB)
+                        if (s_gmin[i] != kInf)
+                            __hip_atomic_fetch_min(&gl[gA + i], s_gmin[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (!grid_sync(a, G, ++epoch, &s_ok)) return;
+                // any live edge left anywhere?
+                uint32_t const any = tid < G ? ald(&a.any[tid]) : 0u;
+                if (!__syncthreads_or(int(any))) break;
+                uint64_t const tp2 = wall_clock64();
+                t_col += tp2 - tp0;
+            }
+            // ---- collect: each graph's frontier of its level
+            {
+                uint64_t const tp0 = wall_clock64();
+                if (B == 0 && tid == 0) ast(&ctl->ovf_n[(X + 1) % 3], 0u);
+                Out const out = out_for(X);
+                uint32_t const cnt = a.live_cnt[B];
+                CrossBuf xb{s_xl[wv], 0, ws};
+                constexpr int U = 4;
+                for (uint32_t base = seg0; base < seg0 + cnt; base += kMB * U) {
+                    xb.ensure(U * kWave, X, a, out);
+                    uint32_t e[U];
+                    uint64_t v[U], L[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        uint32_t const i = base + u * kMB + tid;
+                        e[u] = i < seg0 + cnt ? a.live[i] : 0u;
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        uint32_t const i = base + u * kMB + tid;
+                        bool const in = i < seg0 + cnt;
+                        v[u] = in ? ald(&a.sup[e[u]]) : 0ull;
+                        L[u] = in ? ald(&gl[a.egraph[e[u]]]) : kInf;
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        bool const take = v[u] == L[u];
+                        if (take) {
+                            ast(&a.stamp[e[u]], X);
+                            a.res[e[u]] = L[u];
+                        }
+                        xb.put(take, e[u]);
+                    }
+                }
+                if ((tid & (kWave - 1)) == 0) s_xn[wv] = xb.n;
+                __syncthreads();
+                claim_block(a, X, out, s_xl, s_xn, s_incl, s_off, s_part);
+                close_out(X);
+                uint64_t const tp1 = wall_clock64();
+                if (!grid_sync(a, G, ++epoch, &s_ok)) return;
+                nt = open_list(X);
+                uint64_t const tp2 = wall_clock64();
+                t_col += tp2 - tp0;
+                w_col += tp1 - tp0;
+            }
+            // ---- peel sub-rounds until no graph claims a bloom
+            uint64_t tmin = kInf, dec_written = kInf;
+            for (;;) {
+                uint32_t const nn = peel_sub_round([&](uint32_t bg) { return ald(&gl[bg]); }, tmin, dec_written, 0);
+                if (nn == ~0u) return;
+                if (!nn) break;
+                nt = nn;
+                X++;
+            }
+            X += 2;
+            k++;
+        }
     }
     if (B == 0 && tid == 0) {
         ctl->subrounds = X - 1;
@@ -909,9 +1064,10 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
     }
 }
 
-__global__ void k_msp_init(MspCtl* ctl, uint32_t* arrive, uint32_t n_arrive) {
+__global__ void k_msp_init(MspCtl* ctl, uint32_t* arrive, uint32_t n_arrive, uint64_t* glev, uint32_t n_glev) {
     uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_arrive) arrive[i] = 0;
+    if (i < n_glev) glev[i] = kInf;
     if (i == 0) {
         unsigned long long* w = reinterpret_cast<unsigned long long*>(ctl);
         for (size_t j = 0; j < sizeof(MspCtl) / sizeof(unsigned long long); j++) w[j] = 0;
@@ -945,8 +1101,10 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     unsigned const G = unsigned(c->cu_count);
     if (G == 0 || G >= unsigned(kMB)) return BT_ERR_DEVICE;
     int per_cu = 0;
-    BT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_msp_persist),
-                                                            kMB, 0));
+    bool const batch = dg->egraph != nullptr && dg->ngraph > 0 && c->batch_levels == BT_BATCH_LEVELS_PER_GRAPH;
+    const void* kern = batch ? reinterpret_cast<const void*>(k_msp_persist<true>)
+                             : reinterpret_cast<const void*>(k_msp_persist<false>);
+    BT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kMB, 0));
     if (per_cu < 1) return BT_ERR_DEVICE;
     uint32_t const nbloom = uint32_t(std::max<uint64_t>(ix.NB, 1));
     // claim stamps: R real ones, then the dummies (both sized by the index build)
@@ -960,7 +1118,10 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     size_t const o_slots = o_cnt + up256(sizeof(uint32_t) * 3 * G);
     size_t const o_arrive = o_slots + up256(sizeof(uint64_t) * 4 * G);
     size_t const n_arrive = size_t(G) * kSp;
-    BT_TRY(bt_buf_reserve(ix.seg, o_arrive + sizeof(uint32_t) * n_arrive));
+    size_t const o_any = o_arrive + up256(sizeof(uint32_t) * n_arrive);
+    BT_TRY(bt_buf_reserve(ix.seg, o_any + sizeof(uint32_t) * G));
+    uint32_t const n_glev = batch ? 2 * dg->ngraph : 0u;
+    if (batch) BT_TRY(bt_buf_reserve(ix.glev, sizeof(uint64_t) * n_glev));
     char* seg = static_cast<char*>(ix.seg.ptr);
     MspCtl* ctl = reinterpret_cast<MspCtl*>(dg->ctl);
     MspArgs a{};
@@ -983,6 +1144,13 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     a.slots = reinterpret_cast<uint64_t*>(seg + o_slots);
     a.arrive = reinterpret_cast<uint32_t*>(seg + o_arrive);
     a.ctl = ctl;
+    if (batch) {
+        a.ngraph = dg->ngraph;
+        a.egraph = dg->egraph;
+        a.bgraph = static_cast<const uint32_t*>(ix.bgraph.ptr);
+        a.glev = static_cast<uint64_t*>(ix.glev.ptr);
+        a.any = reinterpret_cast<uint32_t*>(seg + o_any);
+    }
     static bt_buf trace_buf;
     bool const tracing = getenv("BT_PEEL_TRACE_FILE") != nullptr;
     a.diag = (getenv("BT_PEEL_TRACE") || tracing) ? 1u : 0u;
@@ -992,12 +1160,13 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
         a.trace = static_cast<uint64_t*>(trace_buf.ptr);
         BT_HIP_TRY(hipMemsetAsync(a.trace, 0, sizeof(uint64_t) * a.trace_cap, s));
     }
-    k_msp_init<<<grid_for(n_arrive, 256, 1u << 16), 256, 0, s>>>(ctl, a.arrive, uint32_t(n_arrive));
+    k_msp_init<<<grid_for(std::max<size_t>(n_arrive, n_glev), 256, 1u << 16), 256, 0, s>>>(
+        ctl, a.arrive, uint32_t(n_arrive), a.glev, n_glev);
     BT_HIP_TRY(hipGetLastError());
     // events 6..7 bracket the persistent kernel alone
     BT_TRY(tm.start(6));
     void* params[] = {&a};
-    BT_HIP_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_msp_persist), dim3(G), dim3(kMB), params,
+    BT_HIP_TRY(hipLaunchCooperativeKernel(kern, dim3(G), dim3(kMB), params,
                                           0, s));
     BT_TRY(tm.start(7));
     // the level count the reference logs is the largest result

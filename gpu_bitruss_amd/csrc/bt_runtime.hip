@@ -152,7 +152,7 @@ int bt_ctx_destroy(bt_ctx* c) {
     bt_buf_free(c->pool);
     bt_buf_free(c->pinned_dummy);
     bt_buf* ixb[] = {&c->ix.keys0, &c->ix.keys1, &c->ix.vals0, &c->ix.vals1, &c->ix.bid, &c->ix.bstart,
-                     &c->ix.bk, &c->ix.bclaim, &c->ix.inc_off, &c->ix.inc_cur, &c->ix.inc, &c->ix.touched, &c->ix.area,
+                     &c->ix.bk, &c->ix.bclaim, &c->ix.inc_off, &c->ix.inc_cur, &c->ix.inc, &c->ix.touched, &c->ix.area, &c->ix.bgraph, &c->ix.glev,
                      &c->ix.live, &c->ix.seg};
     for (bt_buf* b : ixb) bt_buf_free(*b);
     for (auto& e : c->ev)
@@ -173,6 +173,14 @@ int bt_ctx_set_option(bt_ctx* ctx, int option, int64_t value) {
         case BT_OPT_INDEX_MEM_PCT:
             if (value < 1 || value > 95) return BT_ERR_INVALID;
             ctx->index_mem_frac = double(value) / 100.0;
+            return BT_OK;
+        case BT_OPT_BATCH_LEVELS:
+            if (value != BT_BATCH_LEVELS_PER_GRAPH && value != BT_BATCH_LEVELS_UNION) return BT_ERR_INVALID;
+            ctx->batch_levels = int(value);
+            return BT_OK;
+        case BT_OPT_BATCH_WEDGES:
+            if (value < 0) return BT_ERR_INVALID;
+            ctx->batch_wedges = uint64_t(value);
             return BT_OK;
         default:
             return BT_ERR_INVALID;

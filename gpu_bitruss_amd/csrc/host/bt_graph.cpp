@@ -496,6 +496,90 @@ int bt_graph_check(const bt_host_graph* g) {
     return bad ? BT_ERR_INVALID : BT_OK;
 }
 
+int bt_graph_wedges(const bt_host_graph* g, uint64_t* out) {
+    if (!g || !out) return BT_ERR_INVALID;
+    // W = #{(u, v, w): v in N(u), w in N(v), v < u, w < u} in vertex priority
+    // (degree, id) (cpu_bfc.cpp:72): for each v and each neighbour u above
+    // v, the neighbours of v below u
+    auto below = [&](uint32_t x, uint32_t y) {
+        return g->degrees[x] < g->degrees[y] || (g->degrees[x] == g->degrees[y] && x < y);
+    };
+    uint64_t total = 0;
+#pragma omp parallel reduction(+ : total)
+    {
+        std::vector<uint32_t> nb;
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t vv = 0; vv < int64_t(g->n); vv++) {
+            uint32_t const v = uint32_t(vv);
+            nb.assign(g->neighbors + g->offsets[v], g->neighbors + g->offsets[v + 1]);
+            std::sort(nb.begin(), nb.end(), below);
+            for (size_t i = 0; i < nb.size(); i++)
+                if (below(v, nb[i])) total += i;  // nb[0..i) are below nb[i]
+        }
+    }
+    *out = total;
+    return BT_OK;
+}
+
+int bt_graph_union(const bt_host_graph* gs, uint64_t k, bt_host_graph* u, uint64_t* e_off) {
+    if (!u || !e_off || (k && !gs)) return BT_ERR_INVALID;
+    std::memset(u, 0, sizeof(*u));
+    std::vector<uint64_t> uo(k + 1, 0), lo(k + 1, 0);
+    e_off[0] = 0;
+    for (uint64_t i = 0; i < k; i++) {
+        int const st = bt_graph_check(&gs[i]);
+        if (st != BT_OK) return st;
+        uo[i + 1] = uo[i] + gs[i].u_num;
+        lo[i + 1] = lo[i] + gs[i].l_num;
+        e_off[i + 1] = e_off[i] + gs[i].m;
+    }
+    uint64_t const U = uo[k], L = lo[k], m = e_off[k], n = U + L;
+    if (2 * m >= (1ull << 32) || n >= (1ull << 32) - 1) return BT_ERR_OVERFLOW;
+    u->u_num = uint32_t(U);
+    u->l_num = uint32_t(L);
+    u->u_max = U ? uint32_t(U - 1) : 0;
+    u->m = uint32_t(m);
+    u->n = uint32_t(n);
+    u->neighbors = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t) * std::max<uint64_t>(2 * m, 1)));
+    u->edge_ids = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t) * std::max<uint64_t>(2 * m, 1)));
+    u->edges = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t) * std::max<uint64_t>(2 * m, 1)));
+    u->offsets = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t) * (n + 1)));
+    u->degrees = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t) * std::max<uint64_t>(n, 1)));
+    if (!u->neighbors || !u->edge_ids || !u->edges || !u->offsets || !u->degrees) {
+        bt_graph_release(u);
+        return BT_ERR_NOMEM;
+    }
+    // union id of graph i's vertex x: uppers of all graphs, then lowers
+    auto uid = [&](uint64_t i, uint32_t x) -> uint32_t {
+        return x < gs[i].u_num ? uint32_t(uo[i] + x) : uint32_t(U + lo[i] + (x - gs[i].u_num));
+    };
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t ii = 0; ii < int64_t(k); ii++) {
+        uint64_t const i = uint64_t(ii);
+        for (uint32_t x = 0; x < gs[i].n; x++) u->degrees[uid(i, x)] = gs[i].degrees[x];
+    }
+    u->offsets[0] = 0;
+    for (uint64_t x = 0; x < n; x++) u->offsets[x + 1] = u->offsets[x] + u->degrees[x];
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t ii = 0; ii < int64_t(k); ii++) {
+        uint64_t const i = uint64_t(ii);
+        bt_host_graph const& g = gs[i];
+        uint32_t const eb = uint32_t(e_off[i]);
+        for (uint32_t x = 0; x < g.n; x++) {
+            uint32_t const dst = u->offsets[uid(i, x)];
+            for (uint32_t j = 0; j < g.degrees[x]; j++) {
+                u->neighbors[dst + j] = uid(i, g.neighbors[g.offsets[x] + j]);
+                u->edge_ids[dst + j] = eb + g.edge_ids[g.offsets[x] + j];
+            }
+        }
+        for (uint32_t e = 0; e < g.m; e++) {
+            u->edges[2ull * (eb + e)] = uid(i, g.edges[2ull * e]);
+            u->edges[2ull * (eb + e) + 1] = uid(i, g.edges[2ull * e + 1]);
+        }
+    }
+    return BT_OK;
+}
+
 int bt_gen_uniform(uint32_t nu, uint32_t nv, uint64_t m, uint64_t seed, bt_edge_list* out) {
     if (!out) return BT_ERR_INVALID;
     out->count = 0;

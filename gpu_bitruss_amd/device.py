@@ -37,11 +37,23 @@ class Device:
         _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(
             self.handle, _lib.OPT_MSP_PATH, _lib.MSP_PATHS[path]))
 
+    def set_batch_wedges(self, wedges: int) -> None:
+        """Bound of priority wedges per batch union (0: as one index holds)."""
+        _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(self.handle, _lib.OPT_BATCH_WEDGES, wedges))
+
+    def set_batch_levels(self, mode: str) -> None:
+        """"union" (default) or "per_graph": how msp walks a batch union's levels."""
+        _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(
+            self.handle, _lib.OPT_BATCH_LEVELS, _lib.BATCH_LEVELS[mode]))
+
     def set_index_mem_pct(self, pct: int) -> None:
         _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(self.handle, _lib.OPT_INDEX_MEM_PCT, pct))
 
     def upload(self, g: Graph) -> "DeviceGraph":
         return DeviceGraph(self, g)
+
+    def upload_batch(self, graphs) -> "Batch":
+        return Batch(self, list(graphs))
 
     def last_timings(self) -> dict:
         t = _lib.BtTimings()
@@ -114,6 +126,66 @@ class DeviceGraph:
     def free(self) -> None:
         if self._h:
             _lib.lib().bt_graph_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many graphs resident on the device as one disjoint union (bt_batch);
+    each algorithm runs once over all of them and returns per-graph results
+    equal to the plain ones."""
+
+    def __init__(self, dev: Device, graphs):
+        self.dev = dev
+        self.ms = [g.m for g in graphs]
+        self._h = C.c_void_p()
+        hs = (_lib.BtHostGraph * max(len(graphs), 1))()
+        for i, g in enumerate(graphs):
+            hs[i] = g.host_struct()
+        self._keep = graphs  # host arrays stay alive during the upload
+        _lib.check("bt_batch_upload", _lib.lib().bt_batch_upload(dev.handle, hs, len(graphs), C.byref(self._h)))
+        self._keep = None
+
+    @property
+    def handle(self) -> C.c_void_p:
+        if not self._h:
+            raise RuntimeError("batch is freed")
+        return self._h
+
+    def __len__(self) -> int:
+        return len(self.ms)
+
+    def chunks(self) -> int:
+        """Number of unions the batch is held as."""
+        n = C.c_size_t()
+        _lib.check("bt_batch_chunks", _lib.lib().bt_batch_chunks(self.handle, C.byref(n)))
+        return n.value
+
+    def run(self, algo: str, copy: bool = True):
+        """(per-graph uint64 arrays or None, per-graph scalar dicts)."""
+        k = len(self.ms)
+        res = (_lib.BtBatchResult * max(k, 1))()
+        outs = None
+        arrays = None
+        if copy and algo != "bfc":
+            arrays = [np.zeros(max(m, 1), np.uint64) for m in self.ms]
+            outs = (_lib.u64p * max(k, 1))(*[a.ctypes.data_as(_lib.u64p) for a in arrays])
+        _lib.check("bt_batch_run", _lib.lib().bt_batch_run(self.dev.handle, self.handle, _lib.ALGOS[algo],
+                                                            outs, res))
+        scal = [{"total": r.total, "support_max": r.support_max, "levels": r.levels, "rounds": r.rounds}
+                for r in res[:k]]
+        if arrays is not None:
+            arrays = [a[:m] for a, m in zip(arrays, self.ms)]
+        return arrays, scal
+
+    def free(self) -> None:
+        if self._h:
+            _lib.lib().bt_batch_free(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

@@ -88,6 +88,27 @@ class Graph:
                 "bt_graph_from_edges")
         return g
 
+    def priority_wedges(self) -> int:
+        """Wedges butterfly counting enumerates (bt_graph_wedges)."""
+        w = C.c_uint64()
+        _lib.check("bt_graph_wedges", _lib.lib().bt_graph_wedges(C.byref(self.host_struct()), C.byref(w)))
+        return w.value
+
+    @classmethod
+    def union(cls, graphs) -> tuple["Graph", np.ndarray]:
+        """Disjoint union in the batch layout (bt_graph_union): the upper
+        vertices of every graph, then the lower ones, edge ids offset per
+        graph. Returns (union, edge offsets[k + 1])."""
+        graphs = list(graphs)
+        hs = (_lib.BtHostGraph * max(len(graphs), 1))()
+        for i, g in enumerate(graphs):
+            hs[i] = g.host_struct()
+        e_off = np.zeros(len(graphs) + 1, np.uint64)
+        u = cls()
+        u._take(lambda hg: _lib.lib().bt_graph_union(hs, len(graphs), hg, e_off.ctypes.data_as(_lib.u64p)),
+                "bt_graph_union")
+        return u, e_off
+
     @classmethod
     def generate(cls, workload: str, index: int = 0) -> "Graph":
         """One of the named seeded workloads (see WORKLOADS)."""

@@ -82,9 +82,20 @@ int bt_ctx_last_timings(const bt_ctx* ctx, bt_timings* out);
  *                          reference's wedges; _MERGE / _INDEX force one path
  *                          (_INDEX fails with BT_ERR_BUDGET when it cannot run)
  *   BT_OPT_INDEX_MEM_PCT   share of free device memory the index may use
- *                          (1..95, default 50) */
-enum { BT_OPT_MSP_PATH = 1, BT_OPT_INDEX_MEM_PCT = 2 };
+ *                          (1..95, default 50)
+ *   BT_OPT_BATCH_WEDGES    batches uploaded afterwards put consecutive graphs
+ *                          in one union up to this bound of priority wedges
+ *                          (0, the default: as many as one bloom index holds)
+ *   BT_OPT_BATCH_LEVELS    how msp walks the levels of a batch union:
+ *                          BT_BATCH_LEVELS_UNION (default): the union's levels
+ *                          in order (a graph sits out the levels it does not
+ *                          have); BT_BATCH_LEVELS_PER_GRAPH: in each step
+ *                          every graph peels its own next level (more phases
+ *                          of less work; slower on the measured mixes).
+ *                          Results are the same; only the speed differs. */
+enum { BT_OPT_MSP_PATH = 1, BT_OPT_INDEX_MEM_PCT = 2, BT_OPT_BATCH_WEDGES = 3, BT_OPT_BATCH_LEVELS = 4 };
 enum { BT_MSP_PATH_AUTO = 0, BT_MSP_PATH_MERGE = 1, BT_MSP_PATH_INDEX = 2 };
+enum { BT_BATCH_LEVELS_PER_GRAPH = 0, BT_BATCH_LEVELS_UNION = 1 };
 int bt_ctx_set_option(bt_ctx* ctx, int option, int64_t value);
 
 /* Copy a host CSR (the reference Graph arrays, graph.h:43-51) to the
@@ -116,6 +127,44 @@ int bt_hidx(bt_ctx* ctx, bt_dgraph* dg, uint64_t* bitruss_out, uint32_t* rounds)
  * receives the value the reference logs as "levels" (the largest
  * bitruss number, 0 for a butterfly-free graph). */
 int bt_msp(bt_ctx* ctx, bt_dgraph* dg, uint64_t* bitruss_out, uint32_t* levels);
+
+/* ---- batches: many independent graphs decomposed together --------------
+ *
+ * No counterpart in the reference, which runs one graph per process. A batch
+ * keeps k graphs on the device as disjoint unions (the upper vertices of
+ * every graph, then the lower vertices; edge ids offset per graph) and runs
+ * each algorithm once over each union. Butterflies never cross graphs, and
+ * vertex priorities and edge ids keep their order inside each graph, so the
+ * slice of a batch result that belongs to graph i equals the plain result
+ * of graph i: msp sees the levels of other graphs as empty levels, and the
+ * hidx round count of graph i is the round after its last change. */
+typedef struct bt_batch bt_batch;
+
+enum { BT_ALGO_BFC = 1, BT_ALGO_EBFC = 2, BT_ALGO_HIDX = 3, BT_ALGO_MSP = 4 };
+
+/* Per-graph scalars of a batch run: the values the plain calls return. */
+typedef struct bt_batch_result {
+    uint64_t total;        /* bfc / ebfc: butterflies of the graph        */
+    uint64_t support_max;  /* ebfc                                        */
+    uint32_t levels;       /* msp: as bt_msp's `levels`                   */
+    uint32_t rounds;       /* hidx: as bt_hidx's `rounds`                 */
+} bt_batch_result;
+
+/* Copy k host graphs (an array of k structs) to the device as one batch.
+ * Consecutive graphs share one union while a bound of their bloom records
+ * (sum of C(deg, 2) over their vertices) fits a 32-bit index and the index
+ * memory budget (BT_OPT_INDEX_MEM_PCT); a larger batch is held as several
+ * unions, run one after the other. */
+int bt_batch_upload(bt_ctx* ctx, const bt_host_graph* graphs, size_t k, bt_batch** out);
+int bt_batch_free(bt_batch* b);
+/* Number of unions the batch was split into. */
+int bt_batch_chunks(const bt_batch* b, size_t* chunks);
+
+/* Run `algo` (BT_ALGO_*) on every graph of the batch. `outs` (k pointers, or
+ * NULL) receives the per-edge results of graph i in outs[i] (m_i entries:
+ * ebfc supports, hidx or msp bitruss numbers; a NULL entry skips that
+ * graph). `results` (k entries, or NULL) receives the per-graph scalars. */
+int bt_batch_run(bt_ctx* ctx, bt_batch* b, int algo, uint64_t* const* outs, bt_batch_result* results);
 
 #ifdef __cplusplus
 }

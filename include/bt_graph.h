@@ -90,6 +90,19 @@ int bt_graph_write_bin(const char* path, const bt_host_graph* g);
  * reference's check_graph tests (src/graph/graph.cpp:67-106). */
 int bt_graph_check(const bt_host_graph* g);
 
+/* Priority wedges of a graph: #{(u,v,w): v in N(u), w in N(v), v < u,
+ * w < u} for the vertex priority (degree, id) of cpu_bfc.cpp:72-90, the
+ * number of wedges butterfly counting enumerates (an upper bound of the
+ * bloom records of the peel). O(m log deg) on the host. */
+int bt_graph_wedges(const bt_host_graph* g, uint64_t* out);
+
+/* Disjoint union of k graphs (an array of k structs) in the batch layout:
+ * the upper vertices of graph 0, 1, ..., then the lower vertices of graph
+ * 0, 1, ...; each graph's ids keep their order, its edge ids are offset by
+ * e_off[i] (k + 1 entries, filled here). No counterpart in the reference.
+ * Fails with BT_ERR_OVERFLOW when 2m or n does not fit 32 bits. */
+int bt_graph_union(const bt_host_graph* graphs, uint64_t k, bt_host_graph* out, uint64_t* e_off);
+
 /* Free the arrays of a graph filled by this library and zero it. */
 void bt_graph_release(bt_host_graph* g);
 void bt_edge_list_release(bt_edge_list* e);

@@ -99,12 +99,14 @@ def test_empty_and_butterfly_free(dev):
         assert total == 0 and smax == 0 and not sup.any()
         res, levels = dg.msp()
         assert not res.any() and levels == 0
-        assert not dg.hidx()[0].any()
+        h, rounds = dg.hidx()
+        assert not h.any() and rounds == 1  # one round that changes nothing
         dg.free()
     g = G.from_pairs(np.zeros((0, 2), dtype=np.uint32))
     dg = dev.upload(g)
     assert dg.bfc() == 0
     assert dg.msp()[1] == 0
+    assert dg.hidx()[1] == 1 == oracle.hidx(g, 1)[1]
     dg.free()
 
 
