@@ -8,6 +8,7 @@ from .device import (
     Batch,
     Device,
     DeviceGraph,
+    Server,
     bfc_evpp,
     bfc_vpp,
     bitruss_msp,
@@ -18,6 +19,6 @@ from .device import (
 from ._lib import BitrussError, LIB_PATH, COH_PATH
 
 __all__ = [
-    "Graph", "WORKLOADS", "Batch", "Device", "DeviceGraph", "bfc_vpp", "bfc_evpp", "g_bitruss_hindex",
+    "Graph", "WORKLOADS", "Batch", "Device", "Server", "DeviceGraph", "bfc_vpp", "bfc_evpp", "g_bitruss_hindex",
     "bitruss_msp", "default_device", "device_count", "BitrussError", "LIB_PATH", "COH_PATH",
 ]

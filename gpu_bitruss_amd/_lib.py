@@ -72,6 +72,25 @@ class BtBatchResult(C.Structure):
     ]
 
 
+class BtServeConfig(C.Structure):
+    """struct bt_serve_config (include/bitruss_hip.h)."""
+
+    _fields_ = [
+        ("device", C.c_int),
+        ("budget_bytes", C.c_uint64),
+        ("large_edges", C.c_uint32),
+        ("max_batch", C.c_uint32),
+    ]
+
+
+class BtServeStats(C.Structure):
+    """struct bt_serve_stats (include/bitruss_hip.h)."""
+
+    _fields_ = [(k, C.c_uint64) for k in (
+        "submitted", "completed", "failed", "batches", "batched_requests", "large_runs", "peak_bytes",
+        "in_use_bytes")]
+
+
 class BitrussError(RuntimeError):
     """A non-zero status returned by libbitruss_hip.so."""
 
@@ -137,6 +156,12 @@ def lib() -> C.CDLL:
         "bt_batch_free": (C.c_int, [vp]),
         "bt_batch_chunks": (C.c_int, [vp, C.POINTER(C.c_size_t)]),
         "bt_batch_run": (C.c_int, [vp, vp, C.c_int, C.POINTER(u64p), C.POINTER(BtBatchResult)]),
+        "bt_serve_start": (C.c_int, [C.POINTER(BtServeConfig), C.POINTER(vp)]),
+        "bt_serve_submit": (C.c_int, [vp, C.POINTER(BtHostGraph), C.c_int, C.POINTER(C.c_uint64)]),
+        "bt_serve_poll": (C.c_int, [vp, C.c_uint64, C.POINTER(C.c_int)]),
+        "bt_serve_wait": (C.c_int, [vp, C.c_uint64, u64p, C.POINTER(BtBatchResult), C.POINTER(C.c_double)]),
+        "bt_serve_stats_get": (C.c_int, [vp, C.POINTER(BtServeStats)]),
+        "bt_serve_stop": (C.c_int, [vp]),
     }
     for name, (res, args) in {**sig, **gpu_sig}.items():
         f = getattr(L, name)

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <cstddef>
 #include <cstdint>
 #include <string>
@@ -139,6 +141,11 @@ constexpr uint64_t kNoIndex = ~0ull;
 int bt_msp_impl(bt_dgraph* dg, uint32_t* levels);
 int bt_hidx_impl(bt_dgraph* dg, uint32_t* rounds);
 int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds);
+
+// Cooperative (grid-barrier) kernels take a device one at a time: two of
+// them resident together could each hold CUs that the other's blocks wait
+// for. Held from the launch until the kernel has finished.
+std::mutex& bt_coop_lock(int device);
 
 // Read back a few control words (synchronises the stream).
 int bt_read_ctl(bt_dgraph* dg, int first, int count, uint64_t* out);

@@ -1166,9 +1166,12 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     // events 6..7 bracket the persistent kernel alone
     BT_TRY(tm.start(6));
     void* params[] = {&a};
-    BT_HIP_TRY(hipLaunchCooperativeKernel(kern, dim3(G), dim3(kMB), params,
-                                          0, s));
-    BT_TRY(tm.start(7));
+    {
+        std::lock_guard<std::mutex> coop(bt_coop_lock(c->device));
+        BT_HIP_TRY(hipLaunchCooperativeKernel(kern, dim3(G), dim3(kMB), params, 0, s));
+        BT_TRY(tm.start(7));
+        BT_HIP_TRY(hipStreamSynchronize(s));
+    }
     // the level count the reference logs is the largest result
     BT_HIP_TRY(hipMemsetAsync(&ctl->level_max, 0, sizeof(uint64_t), s));
     if (m)

@@ -52,6 +52,11 @@ int bt_read_ctl(bt_dgraph* dg, int first, int count, uint64_t* out) {
     return BT_OK;
 }
 
+std::mutex& bt_coop_lock(int device) {
+    static std::mutex locks[64];
+    return locks[static_cast<unsigned>(device) % 64];
+}
+
 namespace {
 
 template <typename T>

@@ -1,0 +1,316 @@
+// Serving: a request loop with continuous batching on one device.
+//
+// No counterpart in the reference (one graph per process). Requests carry a
+// host graph and an algorithm; the server copies the graph, estimates the
+// device memory the request needs, and routes it:
+//   - small graphs (fewer than cfg.large_edges edges) queue for the batch
+//     worker, which takes every pending small request of the oldest
+//     request's algorithm that fits the memory budget (up to cfg.max_batch)
+//     and runs them as one batch (bt_batch_*, one union launch sequence);
+//     requests that arrive meanwhile form the next batch;
+//   - large graphs queue for the large worker, which runs them one at a
+//     time on the plain path (bt_graph_upload + the algorithm).
+// Each worker has its own context (HIP stream), so a large request's
+// counting and index building overlap the batches on the device; the
+// persistent peel kernels themselves take the device one at a time (see
+// bt_coop_lock). Admission: a request starts only when the estimated bytes
+// of all running requests plus its own stay within cfg.budget_bytes;
+// requests wait in arrival order per queue.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/bitruss_hip.h"
+#include "bt_internal.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+    uint64_t ticket = 0;
+    int algo = 0;
+    bt_host_graph g{};                  // owned copy
+    std::vector<uint32_t> arrays;       // storage of g's arrays
+    uint64_t est_bytes = 0;
+    std::vector<uint64_t> out;          // per-edge result
+    bt_batch_result res{};
+    int status = BT_OK;
+    bool done = false;
+    Clock::time_point t_submit, t_done;
+};
+
+using RequestPtr = std::shared_ptr<Request>;
+
+// Device bytes a request may hold: the resident graph and its scratch
+// (~137 B per edge, 40 B per vertex), the bloom index (~44 B per priority
+// wedge) and the peel's fixed control buffers.
+uint64_t estimate_bytes(const bt_host_graph& g, uint64_t wedges) {
+    return 140ull * g.m + 48ull * g.n + 48ull * wedges + (64ull << 20);
+}
+
+}  // namespace
+
+struct bt_server {
+    bt_serve_config cfg{};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<RequestPtr> small_q, large_q;
+    std::unordered_map<uint64_t, RequestPtr> all;
+    uint64_t next_ticket = 1;
+    uint64_t in_use = 0;
+    bool stopping = false;
+    bt_serve_stats stats{};
+    bt_ctx* ctx_small = nullptr;
+    bt_ctx* ctx_large = nullptr;
+    std::thread thr_small, thr_large;
+
+    // takes the next group of requests from q that fits the budget (caller
+    // holds mu); empty when the front does not fit yet
+    std::vector<RequestPtr> take(std::deque<RequestPtr>& q, bool batch) {
+        std::vector<RequestPtr> got;
+        uint64_t add = 0;
+        int const algo = q.empty() ? 0 : q.front()->algo;
+        for (auto it = q.begin(); it != q.end() && got.size() < (batch ? cfg.max_batch : 1u);) {
+            RequestPtr const& r = *it;
+            if (r->algo != algo) {
+                ++it;
+                continue;
+            }
+            if (in_use + add + r->est_bytes > cfg.budget_bytes) break;
+            add += r->est_bytes;
+            got.push_back(r);
+            it = q.erase(it);
+        }
+        in_use += add;
+        if (in_use > stats.peak_bytes) stats.peak_bytes = in_use;
+        return got;
+    }
+
+    void finish(std::vector<RequestPtr>& rs, int st) {
+        std::lock_guard<std::mutex> lk(mu);
+        for (RequestPtr& r : rs) {
+            r->status = r->status == BT_OK ? st : r->status;
+            r->done = true;
+            r->t_done = Clock::now();
+            in_use -= r->est_bytes;
+            stats.completed++;
+            if (r->status != BT_OK) stats.failed++;
+            // the graph copy is no longer needed
+            r->arrays.clear();
+            r->arrays.shrink_to_fit();
+        }
+        cv_done.notify_all();
+        cv_work.notify_all();
+    }
+
+    void run_small() {
+        for (;;) {
+            std::vector<RequestPtr> rs;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                for (;;) {
+                    if (!small_q.empty()) {
+                        rs = take(small_q, true);
+                        if (!rs.empty()) break;
+                    } else if (stopping) {
+                        return;
+                    }
+                    cv_work.wait(lk);
+                }
+                stats.batches++;
+                stats.batched_requests += rs.size();
+            }
+            std::vector<bt_host_graph> gs(rs.size());
+            std::vector<uint64_t*> outs(rs.size());
+            std::vector<bt_batch_result> res(rs.size());
+            for (size_t i = 0; i < rs.size(); i++) {
+                gs[i] = rs[i]->g;
+                rs[i]->out.assign(rs[i]->algo == BT_ALGO_BFC ? 0 : rs[i]->g.m, 0);
+                outs[i] = rs[i]->out.empty() ? nullptr : rs[i]->out.data();
+            }
+            bt_batch* b = nullptr;
+            int st = bt_batch_upload(ctx_small, gs.data(), gs.size(), &b);
+            if (st == BT_OK) st = bt_batch_run(ctx_small, b, rs.front()->algo, outs.data(), res.data());
+            if (b) bt_batch_free(b);
+            if (st == BT_OK)
+                for (size_t i = 0; i < rs.size(); i++) rs[i]->res = res[i];
+            finish(rs, st);
+        }
+    }
+
+    void run_large() {
+        for (;;) {
+            std::vector<RequestPtr> rs;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                for (;;) {
+                    if (!large_q.empty()) {
+                        rs = take(large_q, false);
+                        if (!rs.empty()) break;
+                    } else if (stopping) {
+                        return;
+                    }
+                    cv_work.wait(lk);
+                }
+                stats.large_runs++;
+            }
+            Request& r = *rs.front();
+            bt_dgraph* dg = nullptr;
+            int st = bt_graph_upload(ctx_large, &r.g, &dg);
+            if (st == BT_OK) {
+                r.out.assign(r.algo == BT_ALGO_BFC ? 0 : r.g.m, 0);
+                uint64_t* out = r.out.empty() ? nullptr : r.out.data();
+                switch (r.algo) {
+                    case BT_ALGO_BFC: st = bt_bfc(ctx_large, dg, &r.res.total); break;
+                    case BT_ALGO_EBFC: st = bt_ebfc(ctx_large, dg, out, &r.res.total, &r.res.support_max); break;
+                    case BT_ALGO_HIDX: st = bt_hidx(ctx_large, dg, out, &r.res.rounds); break;
+                    default: st = bt_msp(ctx_large, dg, out, &r.res.levels); break;
+                }
+                bt_graph_free(dg);
+            }
+            finish(rs, st);
+        }
+    }
+};
+
+extern "C" {
+
+int bt_serve_start(const bt_serve_config* cfg, bt_server** out) {
+    if (!out) return BT_ERR_INVALID;
+    *out = nullptr;
+    bt_server* s = new (std::nothrow) bt_server();
+    if (!s) return BT_ERR_NOMEM;
+    if (cfg) s->cfg = *cfg;
+    if (!s->cfg.budget_bytes) s->cfg.budget_bytes = 64ull << 30;
+    if (!s->cfg.large_edges) s->cfg.large_edges = 1000000;
+    if (!s->cfg.max_batch) s->cfg.max_batch = 1024;
+    int st = bt_ctx_create(s->cfg.device, &s->ctx_small);
+    if (st == BT_OK) st = bt_ctx_create(s->cfg.device, &s->ctx_large);
+    if (st != BT_OK) {
+        if (s->ctx_small) bt_ctx_destroy(s->ctx_small);
+        delete s;
+        return st;
+    }
+    try {
+        s->thr_small = std::thread([s] { s->run_small(); });
+        s->thr_large = std::thread([s] { s->run_large(); });
+    } catch (...) {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stopping = true;
+        }
+        s->cv_work.notify_all();
+        if (s->thr_small.joinable()) s->thr_small.join();
+        bt_ctx_destroy(s->ctx_small);
+        bt_ctx_destroy(s->ctx_large);
+        delete s;
+        return BT_ERR_NOMEM;
+    }
+    *out = s;
+    return BT_OK;
+}
+
+int bt_serve_submit(bt_server* s, const bt_host_graph* g, int algo, uint64_t* ticket) {
+    if (!s || !g || !ticket || algo < BT_ALGO_BFC || algo > BT_ALGO_MSP) return BT_ERR_INVALID;
+    BT_TRY(bt_graph_check(g));
+    uint64_t wedges = 0;
+    BT_TRY(bt_graph_wedges(g, &wedges));
+    RequestPtr r;
+    try {
+        r = std::make_shared<Request>();
+        size_t const m2 = 2ull * g->m, n = g->n;
+        r->arrays.resize(3 * m2 + 2 * n + 1);
+        uint32_t* p = r->arrays.data();
+        auto put = [&](const uint32_t* src, size_t cnt) {
+            uint32_t* d = p;
+            if (cnt) std::memcpy(d, src, sizeof(uint32_t) * cnt);
+            p += cnt;
+            return d;
+        };
+        r->g = *g;
+        r->g.neighbors = put(g->neighbors, m2);
+        r->g.edges = put(g->edges, m2);
+        r->g.edge_ids = put(g->edge_ids, m2);
+        r->g.degrees = put(g->degrees, n);
+        r->g.offsets = put(g->offsets, n + 1);
+    } catch (...) {
+        return BT_ERR_NOMEM;
+    }
+    r->algo = algo;
+    r->est_bytes = estimate_bytes(*g, wedges);
+    if (r->est_bytes > s->cfg.budget_bytes) return BT_ERR_BUDGET;
+    r->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stopping) return BT_ERR_STATE;
+        r->ticket = s->next_ticket++;
+        s->all[r->ticket] = r;
+        (g->m >= s->cfg.large_edges ? s->large_q : s->small_q).push_back(r);
+        s->stats.submitted++;
+        *ticket = r->ticket;
+    }
+    s->cv_work.notify_all();
+    return BT_OK;
+}
+
+int bt_serve_poll(bt_server* s, uint64_t ticket, int* done) {
+    if (!s || !done) return BT_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(ticket);
+    if (it == s->all.end()) return BT_ERR_INVALID;
+    *done = it->second->done ? 1 : 0;
+    return BT_OK;
+}
+
+int bt_serve_wait(bt_server* s, uint64_t ticket, uint64_t* out, bt_batch_result* result, double* latency_ms) {
+    if (!s) return BT_ERR_INVALID;
+    RequestPtr r;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->all.find(ticket);
+        if (it == s->all.end()) return BT_ERR_INVALID;
+        r = it->second;
+        s->cv_done.wait(lk, [&] { return r->done; });
+        s->all.erase(it);
+    }
+    if (r->status != BT_OK) return r->status;
+    if (out && !r->out.empty()) std::memcpy(out, r->out.data(), sizeof(uint64_t) * r->out.size());
+    if (result) *result = r->res;
+    if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    return BT_OK;
+}
+
+int bt_serve_stats_get(bt_server* s, bt_serve_stats* st) {
+    if (!s || !st) return BT_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    *st = s->stats;
+    st->in_use_bytes = s->in_use;
+    return BT_OK;
+}
+
+int bt_serve_stop(bt_server* s) {
+    if (!s) return BT_ERR_INVALID;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stopping = true;
+    }
+    s->cv_work.notify_all();
+    if (s->thr_small.joinable()) s->thr_small.join();
+    if (s->thr_large.joinable()) s->thr_large.join();
+    bt_ctx_destroy(s->ctx_small);
+    bt_ctx_destroy(s->ctx_large);
+    delete s;
+    return BT_OK;
+}
+
+}  // extern "C"

@@ -195,6 +195,64 @@ class Batch:
             pass
 
 
+class Server:
+    """Request loop with continuous batching (bt_serve_*): small graphs are
+    decomposed together in batches, large ones one at a time, under a
+    device-memory admission budget."""
+
+    def __init__(self, device: int = 0, budget_bytes: int = 0, large_edges: int = 0, max_batch: int = 0):
+        cfg = _lib.BtServeConfig(device, budget_bytes, large_edges, max_batch)
+        self._h = C.c_void_p()
+        _lib.check("bt_serve_start", _lib.lib().bt_serve_start(C.byref(cfg), C.byref(self._h)))
+        self._m = {}
+
+    def submit(self, g: Graph, algo: str = "msp") -> int:
+        t = C.c_uint64()
+        _lib.check("bt_serve_submit", _lib.lib().bt_serve_submit(self._h, C.byref(g.host_struct()),
+                                                                  _lib.ALGOS[algo], C.byref(t)))
+        self._m[t.value] = (g.m, algo)
+        return t.value
+
+    def done(self, ticket: int) -> bool:
+        d = C.c_int()
+        _lib.check("bt_serve_poll", _lib.lib().bt_serve_poll(self._h, ticket, C.byref(d)))
+        return bool(d.value)
+
+    def wait(self, ticket: int):
+        """(per-edge uint64 array or None, scalars dict, latency in ms)."""
+        m, algo = self._m.pop(ticket)
+        out = np.zeros(max(m, 1), np.uint64) if algo != "bfc" else None
+        res = _lib.BtBatchResult()
+        lat = C.c_double()
+        _lib.check("bt_serve_wait", _lib.lib().bt_serve_wait(
+            self._h, ticket, out.ctypes.data_as(_lib.u64p) if out is not None else None, C.byref(res),
+            C.byref(lat)))
+        scal = {"total": res.total, "support_max": res.support_max, "levels": res.levels, "rounds": res.rounds}
+        return (out[:m] if out is not None else None), scal, lat.value
+
+    def stats(self) -> dict:
+        st = _lib.BtServeStats()
+        _lib.check("bt_serve_stats_get", _lib.lib().bt_serve_stats_get(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def stop(self) -> None:
+        if self._h:
+            _lib.lib().bt_serve_stop(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.stop()
+
+    def __del__(self):
+        try:
+            self.stop()
+        except Exception:
+            pass
+
+
 _default: Device | None = None
 
 

@@ -166,6 +166,52 @@ int bt_batch_chunks(const bt_batch* b, size_t* chunks);
  * graph). `results` (k entries, or NULL) receives the per-graph scalars. */
 int bt_batch_run(bt_ctx* ctx, bt_batch* b, int algo, uint64_t* const* outs, bt_batch_result* results);
 
+/* ---- serving: a request loop with continuous batching ------------------
+ *
+ * No counterpart in the reference. A server owns two contexts on one device
+ * and two worker threads. Small requests (fewer than large_edges edges) are
+ * decomposed together: the batch worker takes every pending small request
+ * of the oldest request's algorithm (up to max_batch) and runs them as one
+ * batch, while requests that arrive meanwhile wait for the next batch.
+ * Large requests run one at a time on the plain path in the other worker;
+ * the two workers' non-persistent kernels overlap on the device. A request
+ * is admitted when the estimated device bytes of the running requests plus
+ * its own fit budget_bytes (the graph, ~140 B per edge, and its bloom
+ * index, ~48 B per priority wedge); a request that alone exceeds the budget
+ * is refused with BT_ERR_BUDGET. Results are those of the plain calls. */
+typedef struct bt_server bt_server;
+
+typedef struct bt_serve_config {
+    int device;
+    uint64_t budget_bytes;  /* admission budget (0: 64 GiB)                  */
+    uint32_t large_edges;   /* edges from which a request runs alone
+                               (0: 1,000,000)                               */
+    uint32_t max_batch;     /* requests per batch at most (0: 1024)         */
+} bt_serve_config;
+
+typedef struct bt_serve_stats {
+    uint64_t submitted, completed, failed;
+    uint64_t batches, batched_requests, large_runs;
+    uint64_t peak_bytes;    /* largest admitted estimate in flight          */
+    uint64_t in_use_bytes;  /* admitted estimate in flight now              */
+} bt_serve_stats;
+
+int bt_serve_start(const bt_serve_config* cfg, bt_server** out);
+/* Queue a request; the graph is copied, so the caller's arrays may be
+ * released on return. *ticket names the request. */
+int bt_serve_submit(bt_server* s, const bt_host_graph* g, int algo, uint64_t* ticket);
+/* *done = 1 once the request has finished (successfully or not). */
+int bt_serve_poll(bt_server* s, uint64_t ticket, int* done);
+/* Block until the request finishes, then return its status and copy its
+ * per-edge result (m entries; not for BT_ALGO_BFC) to out (may be NULL),
+ * its scalars to result and the time from submit to completion to
+ * latency_ms (both may be NULL). The ticket is released. */
+int bt_serve_wait(bt_server* s, uint64_t ticket, uint64_t* out, bt_batch_result* result, double* latency_ms);
+int bt_serve_stats_get(bt_server* s, bt_serve_stats* stats);
+/* Finish every queued request, then stop the workers and free the server
+ * (tickets not waited for are dropped). */
+int bt_serve_stop(bt_server* s);
+
 #ifdef __cplusplus
 }
 #endif
