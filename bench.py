@@ -7,6 +7,8 @@ expected degree cap 5,000, seed 1, with the CSR already resident in HBM and
 the results left on the device.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py --workload mix [--graphs 4096] [--check]   # batched vs plain
+    python bench.py --workload serve [--requests 400] [--qps 25,100,400]
 
 Prints one JSON line. `value` is GBiD wedges/s = (W_cnt + W_peel) / step
 time, where W_cnt (priority wedges of the support count) and W_peel (wedges
@@ -74,6 +76,129 @@ def cpu_baseline(g, threads: int, seconds: float):
     return secs, levels, complete, st
 
 
+def bench_mix(args) -> int:
+    """The 4,096-graph mix (seeds 1000..5095): msp on all graphs as batches
+    (unions) against the plain path graph by graph, both with the graphs
+    resident and the results left on the device."""
+    import numpy as np
+
+    import gpu_bitruss_amd as gb
+
+    k = args.graphs
+    log(f"generating {k} mix graphs")
+    gs = [gb.Graph.generate("mix_graph", i) for i in range(k)]
+    m_total = sum(g.m for g in gs)
+    dev = gb.Device(0)
+    b = dev.upload_batch(gs)
+    log(f"batch: {m_total} edges in {b.chunks()} unions")
+    for i in range(args.warmup):
+        b.run("msp", copy=False)
+    tb = []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        b.run("msp", copy=False)
+        tb.append(time.perf_counter() - t0)
+        log(f"batch step {i}: {tb[-1]:.2f} s")
+    dgs = [dev.upload(g) for g in gs]
+    for dg in dgs[:16]:
+        dg.msp(copy=False)
+    t0 = time.perf_counter()
+    for i, dg in enumerate(dgs):
+        dg.msp(copy=False)
+        if i % 512 == 511:
+            log(f"plain: {i + 1} graphs, {time.perf_counter() - t0:.1f} s")
+    tp = time.perf_counter() - t0
+    same = None
+    if args.check:
+        arrays, scal = b.run("msp")
+        same = 0
+        for i, dg in enumerate(dgs):
+            r, lv = dg.msp()
+            same += int(np.array_equal(r, arrays[i]) and lv == scal[i]["levels"])
+            if i % 1024 == 1023:
+                log(f"checked {i + 1} graphs")
+        log(f"identical: {same} of {k}")
+    for dg in dgs:
+        dg.free()
+    b.free()
+    dev.close()
+    t = float(np.mean(tb))
+    print(json.dumps({
+        "metric": "graphs/s, batched msp vs plain (4,096-graph mix)", "value": k / t, "unit": "graphs/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * t,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": f"mix: {k} bipartite graphs, seeds 1000..{999 + k}, |U|,|V| log-uniform in "
+                               "[50, 5000], density log-uniform in [0.5%, 5%], even seeds uniform, odd Chung-Lu 2.2",
+                   "m": m_total, "algo": "msp"},
+        "plain_graphs_per_s": k / tp, "plain_s": tp, "speedup": tp / t, "identical": same,
+    }), flush=True)
+    return 0
+
+
+def bench_serve(args) -> int:
+    """Poisson request stream (seed 7): 98% graphs of the mix distribution,
+    2% large Chung-Lu requests; p50/p99 latency and graphs/s per offered QPS,
+    against the plain path on the same requests one after the other."""
+    import threading
+
+    import numpy as np
+
+    import gpu_bitruss_amd as gb
+
+    rng = np.random.default_rng(7)
+    n = args.requests
+    large = rng.random(n) < 0.02
+    idx = rng.integers(0, 4096, n)
+    log(f"generating {n} requests ({int(large.sum())} large)")
+    reqs = [gb.Graph.generate("large_request", int(i)) if big else gb.Graph.generate("mix_graph", int(i))
+            for i, big in zip(idx, large)]
+    dev = gb.Device(0)
+    t0 = time.perf_counter()
+    for g in reqs:
+        dg = dev.upload(g)
+        dg.msp(copy=False)
+        dg.free()
+    t_plain = time.perf_counter() - t0
+    log(f"plain path, one graph after the other (upload included): {t_plain:.1f} s")
+    dev.close()
+    levels = []
+    for qps in [float(q) for q in args.qps.split(",")]:
+        gaps = rng.exponential(1.0 / qps, n)
+        with gb.Server(large_edges=1_000_000) as srv:
+            tickets = [None] * n
+            start = time.perf_counter()
+
+            def feed():
+                t = start
+                for i, g in enumerate(reqs):
+                    t += gaps[i]
+                    d = t - time.perf_counter()
+                    if d > 0:
+                        time.sleep(d)
+                    tickets[i] = srv.submit(g, "msp")
+
+            th = threading.Thread(target=feed)
+            th.start()
+            th.join()
+            lat = np.array([srv.wait(t)[2] for t in tickets])
+            wall = time.perf_counter() - start
+            st = srv.stats()
+        p50, p99 = np.percentile(lat, 50), np.percentile(lat, 99)
+        log(f"qps {qps:g}: p50 {p50:.1f} ms, p99 {p99:.1f} ms, {n / wall:.1f} graphs/s, {st['batches']} batches")
+        levels.append({"qps": qps, "p50_ms": p50, "p99_ms": p99, "graphs_per_s": n / wall,
+                       "batches": st["batches"], "large_runs": st["large_runs"]})
+    best = max(levels, key=lambda x: x["graphs_per_s"])
+    print(json.dumps({
+        "metric": "served graphs/s and p50/p99 latency (Poisson stream, msp)", "value": best["graphs_per_s"],
+        "unit": "graphs/s", "n_gpus": 1, "steps": len(levels), "warmup": 0, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": f"serve: {n} requests, 98% mix graphs, 2% Chung-Lu 2.2 with m in [1M, 5M], "
+                               "budget 64 GiB", "algo": "msp"},
+        "plain_graphs_per_s": n / t_plain, "levels": levels,
+    }), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,11 +207,21 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--workload", default=WORKLOAD, choices=[WORKLOAD, "mix", "serve"],
+                    help="powerlaw_2m (the headline line), mix (batched vs plain) or serve")
+    ap.add_argument("--graphs", type=int, default=4096, help="mix: number of graphs")
+    ap.add_argument("--check", action="store_true", help="mix: compare every graph with the plain path")
+    ap.add_argument("--requests", type=int, default=400, help="serve: requests per offered rate")
+    ap.add_argument("--qps", default="25,100,400", help="serve: offered request rates")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the bitruss path runs on a single MI355X)", file=sys.stderr)
         return 2
+    if args.workload == "mix":
+        return bench_mix(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     import numpy as np
 

@@ -80,6 +80,7 @@ class BtServeConfig(C.Structure):
         ("budget_bytes", C.c_uint64),
         ("large_edges", C.c_uint32),
         ("max_batch", C.c_uint32),
+        ("light_wedges", C.c_uint64),
     ]
 
 
@@ -104,6 +105,7 @@ OPT_INDEX_MEM_PCT = 2
 OPT_BATCH_WEDGES = 3
 OPT_BATCH_LEVELS = 4
 BATCH_LEVELS = {"per_graph": 0, "union": 1}
+OPT_PEEL_BLOCKS = 5
 MSP_PATHS = {"auto": 0, "merge": 1, "index": 2}
 ALGOS = {"bfc": 1, "ebfc": 2, "hidx": 3, "msp": 4}
 

@@ -39,6 +39,7 @@ struct bt_chunk {
     uint32_t* egraph = nullptr;      // device, m: graph of each edge (chunk-local)
     uint32_t* glast = nullptr;       // device, per graph
     uint64_t* red = nullptr;         // device, per graph: sum, max
+    std::vector<std::pair<void*, size_t>> blocks;  // device blocks of the arrays above
 };
 
 struct bt_batch {
@@ -89,12 +90,20 @@ __global__ __launch_bounds__(kRB) void k_seg_reduce(uint32_t k, const uint64_t* 
     }
 }
 
-void chunk_free(bt_chunk& c) {
-    if (c.dg) bt_graph_free(c.dg);
-    void* ptrs[] = {c.d_eoff, c.egraph, c.glast, c.red};
-    for (void* p : ptrs)
-        if (p) (void)hipFree(p);
+void chunk_free(bt_chunk& c, bt_ctx* ctx) {
+    if (c.dg) bt_graph_free(c.dg);  // synchronises the stream
+    for (auto const& b : c.blocks) bt_ctx_release(ctx, b.first, b.second);
     c = bt_chunk{};
+}
+
+template <typename T>
+int chunk_alloc(bt_ctx* ctx, bt_chunk& c, T** p, size_t count) {
+    void* q = nullptr;
+    size_t got = 0;
+    BT_TRY(bt_ctx_alloc(ctx, sizeof(T) * std::max<size_t>(count, 1), &q, &got));
+    c.blocks.emplace_back(q, got);
+    *p = static_cast<T*>(q);
+    return BT_OK;
 }
 
 int chunk_upload(bt_ctx* ctx, const bt_host_graph* graphs, bt_chunk& c) {
@@ -106,12 +115,11 @@ int chunk_upload(bt_ctx* ctx, const bt_host_graph* graphs, bt_chunk& c) {
     bt_graph_release(&u);
     if (st != BT_OK) return st;
     size_t const kk = std::max<size_t>(k, 1), m = c.dg->m;
-    if (hipMalloc(&c.d_eoff, sizeof(uint64_t) * (kk + 1)) != hipSuccess ||
-        hipMalloc(&c.egraph, sizeof(uint32_t) * std::max<size_t>(m, 1)) != hipSuccess ||
-        hipMalloc(&c.glast, sizeof(uint32_t) * kk) != hipSuccess ||
-        hipMalloc(&c.red, sizeof(uint64_t) * 2 * kk) != hipSuccess)
-        return BT_ERR_BUDGET;
     hipStream_t s = ctx->stream;
+    BT_TRY(chunk_alloc(ctx, c, &c.d_eoff, kk + 1));
+    BT_TRY(chunk_alloc(ctx, c, &c.egraph, m));
+    BT_TRY(chunk_alloc(ctx, c, &c.glast, kk));
+    BT_TRY(chunk_alloc(ctx, c, &c.red, 2 * kk));
     BT_HIP_TRY(hipMemcpyAsync(c.d_eoff, c.e_off.data(), sizeof(uint64_t) * (k + 1), hipMemcpyHostToDevice, s));
     if (k) k_fill_egraph<<<unsigned(std::min<size_t>(k, 65535)), 256, 0, s>>>(uint32_t(k), c.d_eoff, c.egraph);
     BT_HIP_TRY(hipGetLastError());
@@ -192,7 +200,7 @@ extern "C" {
 int bt_batch_free(bt_batch* b) {
     if (!b) return BT_ERR_INVALID;
     if (b->ctx) (void)hipSetDevice(b->ctx->device);
-    for (bt_chunk& c : b->chunks) chunk_free(c);
+    for (bt_chunk& c : b->chunks) chunk_free(c, b->ctx);
     delete b;
     return BT_OK;
 }

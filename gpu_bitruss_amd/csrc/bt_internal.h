@@ -4,7 +4,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <condition_variable>
 #include <mutex>
+#include <utility>
+#include <vector>
+#include <map>
 
 #include <cstddef>
 #include <cstdint>
@@ -30,9 +34,14 @@
 void bt_set_hip_error(hipError_t e, const char* what, const char* file, int line);
 
 // A device buffer that grows on demand and is reused between calls.
+struct bt_ctx;
+// A growable device buffer. A buffer owned by a context grows through the
+// context's block cache (the old block goes back to the cache, reused in
+// stream order), so growing it never waits for other contexts' work.
 struct bt_buf {
     void* ptr = nullptr;
     size_t bytes = 0;
+    bt_ctx* owner = nullptr;
 };
 int bt_buf_reserve(bt_buf& b, size_t bytes);
 void bt_buf_free(bt_buf& b);
@@ -80,7 +89,21 @@ struct bt_ctx {
     double index_mem_frac = 0.5;   // share of free device memory the index may use
     uint64_t batch_wedges = 0;     // wedge bound per batch union (0: the index limits)
     int batch_levels = 1;          // BT_BATCH_LEVELS_*: how a union's msp walks levels
+    int peel_blocks = 0;           // CUs the persistent peel uses (0: all)
+    // device blocks given back by freed graphs and batches, reused by the
+    // next ones (hipFree would wait for every stream on the device)
+    std::mutex cache_mu;
+    std::multimap<size_t, void*> cache;
+    size_t cache_bytes = 0;
 };
+
+// Device memory for graphs and batches from the context's cache of freed
+// blocks (best fit within twice the request), else hipMalloc; *got is the
+// block's size, to be passed back to bt_ctx_release.
+int bt_ctx_alloc(bt_ctx* c, size_t bytes, void** p, size_t* got);
+// Give a block back to the context's cache once the work queued on its
+// stream no longer needs it (the caller synchronises the stream first).
+void bt_ctx_release(bt_ctx* c, void* p, size_t bytes);
 
 // Device-resident graph plus the per-graph scratch of the algorithms.
 struct bt_dgraph {
@@ -117,6 +140,7 @@ struct bt_dgraph {
     const uint32_t* egraph = nullptr;  // m
     uint32_t* graph_last = nullptr;    // graphs of the batch
     uint32_t ngraph = 0;               // graphs of the batch (0: not a batch)
+    std::vector<std::pair<void*, size_t>> blocks;  // device blocks of the arrays above
 };
 
 // per-call helpers
@@ -142,10 +166,21 @@ int bt_msp_impl(bt_dgraph* dg, uint32_t* levels);
 int bt_hidx_impl(bt_dgraph* dg, uint32_t* rounds);
 int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds);
 
-// Cooperative (grid-barrier) kernels take a device one at a time: two of
-// them resident together could each hold CUs that the other's blocks wait
-// for. Held from the launch until the kernel has finished.
-std::mutex& bt_coop_lock(int device);
+// Cooperative (grid-barrier) kernels share a device's CUs: each of their
+// blocks needs a CU of its own (LDS), so kernels whose block counts add up
+// to at most the CU count are resident together, while more could each
+// hold CUs that the other's blocks wait for. A kernel takes its CUs before
+// the launch and gives them back when it has finished.
+class bt_cu_lease {
+public:
+    bt_cu_lease(int device, int total_cus, int want);
+    ~bt_cu_lease();
+    bt_cu_lease(const bt_cu_lease&) = delete;
+    bt_cu_lease& operator=(const bt_cu_lease&) = delete;
+
+private:
+    int device_, want_;
+};
 
 // Read back a few control words (synchronises the stream).
 int bt_read_ctl(bt_dgraph* dg, int first, int count, uint64_t* out);

@@ -1098,7 +1098,7 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     bt_index& ix = c->ix;
     // one block per CU keeps the grid barrier short; block 0 watches every
     // other block with one thread
-    unsigned const G = unsigned(c->cu_count);
+    unsigned const G = unsigned(c->peel_blocks > 0 && c->peel_blocks < c->cu_count ? c->peel_blocks : c->cu_count);
     if (G == 0 || G >= unsigned(kMB)) return BT_ERR_DEVICE;
     int per_cu = 0;
     bool const batch = dg->egraph != nullptr && dg->ngraph > 0 && c->batch_levels == BT_BATCH_LEVELS_PER_GRAPH;
@@ -1167,8 +1167,17 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     BT_TRY(tm.start(6));
     void* params[] = {&a};
     {
-        std::lock_guard<std::mutex> coop(bt_coop_lock(c->device));
-        BT_HIP_TRY(hipLaunchCooperativeKernel(kern, dim3(G), dim3(kMB), params, 0, s));
+        bt_cu_lease lease(c->device, c->cu_count, int(G));
+        if (G < unsigned(c->cu_count)) {
+            // Part of the device: the runtime runs cooperative launches one
+            // at a time, so this one is a plain launch. Its blocks are all
+            // resident together: each needs a CU of its own (LDS), the lease
+            // keeps the persistent peels' blocks within the CU count, and
+            // other kernels on the device finish by themselves.
+            BT_HIP_TRY(hipLaunchKernel(kern, dim3(G), dim3(kMB), params, 0, s));
+        } else {
+            BT_HIP_TRY(hipLaunchCooperativeKernel(kern, dim3(G), dim3(kMB), params, 0, s));
+        }
         BT_TRY(tm.start(7));
         BT_HIP_TRY(hipStreamSynchronize(s));
     }

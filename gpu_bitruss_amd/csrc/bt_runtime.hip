@@ -20,12 +20,21 @@ void bt_set_hip_error(hipError_t e, const char* what, const char* file, int line
 
 int bt_buf_reserve(bt_buf& b, size_t bytes) {
     if (bytes <= b.bytes && b.ptr) return BT_OK;
+    size_t const want = bytes ? bytes : 256;
+    if (b.owner) {
+        if (b.ptr) bt_ctx_release(b.owner, b.ptr, b.bytes);
+        b.ptr = nullptr;
+        b.bytes = 0;
+        size_t got = 0;
+        BT_TRY(bt_ctx_alloc(b.owner, want, &b.ptr, &got));
+        b.bytes = got;
+        return BT_OK;
+    }
     if (b.ptr) {
         BT_HIP_TRY(hipFree(b.ptr));
         b.ptr = nullptr;
         b.bytes = 0;
     }
-    size_t const want = bytes ? bytes : 256;
     BT_HIP_TRY(hipMalloc(&b.ptr, want));
     b.bytes = want;
     return BT_OK;
@@ -52,16 +61,82 @@ int bt_read_ctl(bt_dgraph* dg, int first, int count, uint64_t* out) {
     return BT_OK;
 }
 
-std::mutex& bt_coop_lock(int device) {
-    static std::mutex locks[64];
-    return locks[static_cast<unsigned>(device) % 64];
+int bt_ctx_alloc(bt_ctx* c, size_t bytes, void** p, size_t* got) {
+    {
+        std::lock_guard<std::mutex> lk(c->cache_mu);
+        auto it = c->cache.lower_bound(bytes);
+        if (it != c->cache.end() && it->first <= 2 * bytes) {
+            *p = it->second;
+            *got = it->first;  // the block keeps its own size
+            c->cache_bytes -= it->first;
+            c->cache.erase(it);
+            return BT_OK;
+        }
+    }
+    BT_HIP_TRY(hipMalloc(p, bytes));
+    *got = bytes;
+    return BT_OK;
+}
+
+void bt_ctx_release(bt_ctx* c, void* p, size_t bytes) {
+    if (!p) return;
+    if (!c) {
+        (void)hipFree(p);
+        return;
+    }
+    constexpr size_t kCacheCap = size_t(32) << 30;  // blocks kept per context at most
+    {
+        std::lock_guard<std::mutex> lk(c->cache_mu);
+        if (c->cache_bytes + bytes <= kCacheCap) {
+            c->cache.emplace(bytes, p);
+            c->cache_bytes += bytes;
+            return;
+        }
+    }
+    (void)hipFree(p);
+}
+
+namespace {
+struct CuPool {
+    std::mutex mu;
+    std::condition_variable cv;
+    int used = 0;
+};
+CuPool& cu_pool(int device) {
+    static CuPool pools[64];
+    return pools[static_cast<unsigned>(device) % 64];
+}
+}  // namespace
+
+bt_cu_lease::bt_cu_lease(int device, int total_cus, int want) : device_(device), want_(want) {
+    CuPool& p = cu_pool(device_);
+    std::unique_lock<std::mutex> lk(p.mu);
+    p.cv.wait(lk, [&] { return p.used == 0 || p.used + want_ <= total_cus; });
+    p.used += want_;
+}
+
+bt_cu_lease::~bt_cu_lease() {
+    CuPool& p = cu_pool(device_);
+    {
+        std::lock_guard<std::mutex> lk(p.mu);
+        p.used -= want_;
+    }
+    p.cv.notify_all();
 }
 
 namespace {
 
+// Graph arrays come from the context's block cache, so dropping a graph
+// does not wait for the whole device (as hipFree does) while another
+// context's kernels run.
 template <typename T>
-int dalloc(T** p, size_t count) {
-    BT_HIP_TRY(hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (count ? count : 1)));
+int dalloc(bt_dgraph* dg, T** p, size_t count) {
+    size_t const bytes = sizeof(T) * (count ? count : 1);
+    void* q = nullptr;
+    size_t got = 0;
+    BT_TRY(bt_ctx_alloc(dg->ctx, bytes, &q, &got));
+    dg->blocks.emplace_back(q, got);
+    *p = static_cast<T*>(q);
     return BT_OK;
 }
 
@@ -141,6 +216,11 @@ int bt_ctx_create(int device, bt_ctx** out) {
             return BT_ERR_DEVICE;
         }
     }
+    for (bt_buf* b : {&c->sort_tmp, &c->pool, &c->pinned_dummy, &c->ix.keys0, &c->ix.keys1, &c->ix.vals0,
+                      &c->ix.vals1, &c->ix.bid, &c->ix.bstart, &c->ix.bk, &c->ix.bclaim, &c->ix.inc_off,
+                      &c->ix.inc_cur, &c->ix.inc, &c->ix.touched, &c->ix.area, &c->ix.bgraph, &c->ix.glev,
+                      &c->ix.live, &c->ix.seg})
+        b->owner = c;
     if (hipHostMalloc(reinterpret_cast<void**>(&c->h_pinned), 64 * sizeof(uint64_t)) != hipSuccess) {
         bt_ctx_destroy(c);
         return BT_ERR_DEVICE;
@@ -160,6 +240,8 @@ int bt_ctx_destroy(bt_ctx* c) {
                      &c->ix.bk, &c->ix.bclaim, &c->ix.inc_off, &c->ix.inc_cur, &c->ix.inc, &c->ix.touched, &c->ix.area, &c->ix.bgraph, &c->ix.glev,
                      &c->ix.live, &c->ix.seg};
     for (bt_buf* b : ixb) bt_buf_free(*b);
+    for (auto const& kv : c->cache) (void)hipFree(kv.second);
+    c->cache.clear();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
@@ -183,6 +265,10 @@ int bt_ctx_set_option(bt_ctx* ctx, int option, int64_t value) {
             if (value != BT_BATCH_LEVELS_PER_GRAPH && value != BT_BATCH_LEVELS_UNION) return BT_ERR_INVALID;
             ctx->batch_levels = int(value);
             return BT_OK;
+        case BT_OPT_PEEL_BLOCKS:
+            if (value < 0 || value > ctx->cu_count) return BT_ERR_INVALID;
+            ctx->peel_blocks = int(value);
+            return BT_OK;
         case BT_OPT_BATCH_WEDGES:
             if (value < 0) return BT_ERR_INVALID;
             ctx->batch_wedges = uint64_t(value);
@@ -204,12 +290,7 @@ int bt_graph_free(bt_dgraph* dg) {
         (void)hipSetDevice(dg->ctx->device);
         (void)hipStreamSynchronize(dg->ctx->stream);
     }
-    void* ptrs[] = {dg->nbr,  dg->off,   dg->deg,   dg->edges, dg->eids,  dg->rank,
-                    dg->r_off, dg->r_nbr, dg->r_eid, dg->r_lo,  dg->r_p,   dg->r_w, dg->r_up,
-                    dg->sup,  dg->res,   dg->aux,   dg->state, dg->list0, dg->list1,
-                    dg->ctl,  dg->r_key, dg->r_key2, dg->r_val2};
-    for (void* p : ptrs)
-        if (p) (void)hipFree(p);
+    for (auto const& b : dg->blocks) bt_ctx_release(dg->ctx, b.first, b.second);
     delete dg;
     return BT_OK;
 }
@@ -233,29 +314,29 @@ int bt_graph_upload(bt_ctx* ctx, const bt_host_graph* g, bt_dgraph** out) {
     auto A = [&](int s) {
         if (st == BT_OK) st = s;
     };
-    A(dalloc(&dg->nbr, 2 * m));
-    A(dalloc(&dg->off, n + 1));
-    A(dalloc(&dg->deg, n));
-    A(dalloc(&dg->edges, 2 * m));
-    A(dalloc(&dg->eids, 2 * m));
-    A(dalloc(&dg->rank, n));
-    A(dalloc(&dg->r_off, n + 1));
-    A(dalloc(&dg->r_nbr, 2 * m));
-    A(dalloc(&dg->r_eid, 2 * m));
-    A(dalloc(&dg->r_lo, n));
-    A(dalloc(&dg->r_up, n));
-    A(dalloc(&dg->r_p, 2 * m));
-    A(dalloc(&dg->r_w, n + 1));
-    A(dalloc(&dg->sup, m));
-    A(dalloc(&dg->res, m));
-    A(dalloc(&dg->aux, m));
-    A(dalloc(&dg->state, m));
-    A(dalloc(&dg->list0, m + n + 1));
-    A(dalloc(&dg->list1, m + n + 1));
-    A(dalloc(&dg->ctl, 128));
-    A(dalloc(&dg->r_key, 2 * m + n));
-    A(dalloc(&dg->r_key2, 2 * m + n));
-    A(dalloc(&dg->r_val2, 2 * m + n));
+    A(dalloc(dg, &dg->nbr, 2 * m));
+    A(dalloc(dg, &dg->off, n + 1));
+    A(dalloc(dg, &dg->deg, n));
+    A(dalloc(dg, &dg->edges, 2 * m));
+    A(dalloc(dg, &dg->eids, 2 * m));
+    A(dalloc(dg, &dg->rank, n));
+    A(dalloc(dg, &dg->r_off, n + 1));
+    A(dalloc(dg, &dg->r_nbr, 2 * m));
+    A(dalloc(dg, &dg->r_eid, 2 * m));
+    A(dalloc(dg, &dg->r_lo, n));
+    A(dalloc(dg, &dg->r_up, n));
+    A(dalloc(dg, &dg->r_p, 2 * m));
+    A(dalloc(dg, &dg->r_w, n + 1));
+    A(dalloc(dg, &dg->sup, m));
+    A(dalloc(dg, &dg->res, m));
+    A(dalloc(dg, &dg->aux, m));
+    A(dalloc(dg, &dg->state, m));
+    A(dalloc(dg, &dg->list0, m + n + 1));
+    A(dalloc(dg, &dg->list1, m + n + 1));
+    A(dalloc(dg, &dg->ctl, 128));
+    A(dalloc(dg, &dg->r_key, 2 * m + n));
+    A(dalloc(dg, &dg->r_key2, 2 * m + n));
+    A(dalloc(dg, &dg->r_val2, 2 * m + n));
     if (st != BT_OK) {
         bt_graph_free(dg);
         return st == BT_ERR_DEVICE ? BT_ERR_BUDGET : st;

@@ -2,22 +2,26 @@
 //
 // No counterpart in the reference (one graph per process). Requests carry a
 // host graph and an algorithm; the server copies the graph, estimates the
-// device memory the request needs, and routes it:
-//   - small graphs (fewer than cfg.large_edges edges) queue for the batch
-//     worker, which takes every pending small request of the oldest
-//     request's algorithm that fits the memory budget (up to cfg.max_batch)
-//     and runs them as one batch (bt_batch_*, one union launch sequence);
-//     requests that arrive meanwhile form the next batch;
-//   - large graphs queue for the large worker, which runs them one at a
-//     time on the plain path (bt_graph_upload + the algorithm).
-// Each worker has its own context (HIP stream), so a large request's
-// counting and index building overlap the batches on the device; the
-// persistent peel kernels themselves take the device one at a time (see
-// bt_coop_lock). Admission: a request starts only when the estimated bytes
-// of all running requests plus its own stay within cfg.budget_bytes;
-// requests wait in arrival order per queue.
+// device memory the request needs, and routes it to one of three lanes:
+//   - light: small graphs with few priority wedges (below
+//     cfg.light_wedges); their worker takes every pending light request of
+//     the oldest request's algorithm that fits the memory budget (up to
+//     cfg.max_batch) and runs them as one batch (bt_batch_*); requests that
+//     arrive meanwhile form the next batch. Its persistent peel occupies a
+//     quarter of the CUs, so light requests are not held up by heavy ones;
+//   - heavy: the other small graphs, batched the same way on the other
+//     three quarters of the CUs;
+//   - large: graphs with at least cfg.large_edges edges, run one at a time
+//     on the plain path (bt_graph_upload + the algorithm), sharing the
+//     heavy lane's CUs.
+// Each lane has its own context (HIP stream); the lanes' kernels overlap on
+// the device, and their persistent peels run side by side on disjoint CU
+// counts (bt_cu_lease). Admission: a request starts only when the estimated
+// bytes of all running requests plus its own stay within cfg.budget_bytes;
+// requests wait in arrival order per lane.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -42,6 +46,7 @@ struct Request {
     bt_host_graph g{};                  // owned copy
     std::vector<uint32_t> arrays;       // storage of g's arrays
     uint64_t est_bytes = 0;
+    uint64_t run_bytes = 0;             // the run's fixed share, on one request of it
     std::vector<uint64_t> out;          // per-edge result
     bt_batch_result res{};
     int status = BT_OK;
@@ -52,11 +57,13 @@ struct Request {
 using RequestPtr = std::shared_ptr<Request>;
 
 // Device bytes a request may hold: the resident graph and its scratch
-// (~137 B per edge, 40 B per vertex), the bloom index (~44 B per priority
-// wedge) and the peel's fixed control buffers.
+// (~137 B per edge, 40 B per vertex) and the bloom index (~44 B per
+// priority wedge). Each run (a batch or a large request) adds the peel's
+// control buffers once.
 uint64_t estimate_bytes(const bt_host_graph& g, uint64_t wedges) {
-    return 140ull * g.m + 48ull * g.n + 48ull * wedges + (64ull << 20);
+    return 140ull * g.m + 48ull * g.n + 48ull * wedges;
 }
+constexpr uint64_t kRunBytes = 64ull << 20;
 
 }  // namespace
 
@@ -64,21 +71,22 @@ struct bt_server {
     bt_serve_config cfg{};
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
-    std::deque<RequestPtr> small_q, large_q;
+    std::deque<RequestPtr> light_q, heavy_q, large_q;
     std::unordered_map<uint64_t, RequestPtr> all;
     uint64_t next_ticket = 1;
     uint64_t in_use = 0;
     bool stopping = false;
     bt_serve_stats stats{};
-    bt_ctx* ctx_small = nullptr;
+    bt_ctx* ctx_light = nullptr;
+    bt_ctx* ctx_heavy = nullptr;
     bt_ctx* ctx_large = nullptr;
-    std::thread thr_small, thr_large;
+    std::thread thr_light, thr_heavy, thr_large;
 
     // takes the next group of requests from q that fits the budget (caller
     // holds mu); empty when the front does not fit yet
     std::vector<RequestPtr> take(std::deque<RequestPtr>& q, bool batch) {
         std::vector<RequestPtr> got;
-        uint64_t add = 0;
+        uint64_t add = kRunBytes;
         int const algo = q.empty() ? 0 : q.front()->algo;
         for (auto it = q.begin(); it != q.end() && got.size() < (batch ? cfg.max_batch : 1u);) {
             RequestPtr const& r = *it;
@@ -91,8 +99,10 @@ struct bt_server {
             got.push_back(r);
             it = q.erase(it);
         }
+        if (got.empty()) return got;
         in_use += add;
         if (in_use > stats.peak_bytes) stats.peak_bytes = in_use;
+        got.front()->run_bytes = kRunBytes;  // given back with the first request
         return got;
     }
 
@@ -102,7 +112,7 @@ struct bt_server {
             r->status = r->status == BT_OK ? st : r->status;
             r->done = true;
             r->t_done = Clock::now();
-            in_use -= r->est_bytes;
+            in_use -= r->est_bytes + r->run_bytes;
             stats.completed++;
             if (r->status != BT_OK) stats.failed++;
             // the graph copy is no longer needed
@@ -113,23 +123,30 @@ struct bt_server {
         cv_work.notify_all();
     }
 
-    void run_small() {
+    // waits for work in q (or the end); false when the server stops
+    bool next(std::deque<RequestPtr>& q, bool batch, std::vector<RequestPtr>& rs) {
+        std::unique_lock<std::mutex> lk(mu);
         for (;;) {
-            std::vector<RequestPtr> rs;
-            {
-                std::unique_lock<std::mutex> lk(mu);
-                for (;;) {
-                    if (!small_q.empty()) {
-                        rs = take(small_q, true);
-                        if (!rs.empty()) break;
-                    } else if (stopping) {
-                        return;
-                    }
-                    cv_work.wait(lk);
-                }
-                stats.batches++;
-                stats.batched_requests += rs.size();
+            if (!q.empty()) {
+                rs = take(q, batch);
+                if (!rs.empty()) break;
+            } else if (stopping) {
+                return false;
             }
+            cv_work.wait(lk);
+        }
+        if (batch) {
+            stats.batches++;
+            stats.batched_requests += rs.size();
+        } else {
+            stats.large_runs++;
+        }
+        return true;
+    }
+
+    void run_batches(std::deque<RequestPtr>& q, bt_ctx* ctx) {
+        std::vector<RequestPtr> rs;
+        while (next(q, true, rs)) {
             std::vector<bt_host_graph> gs(rs.size());
             std::vector<uint64_t*> outs(rs.size());
             std::vector<bt_batch_result> res(rs.size());
@@ -139,8 +156,8 @@ struct bt_server {
                 outs[i] = rs[i]->out.empty() ? nullptr : rs[i]->out.data();
             }
             bt_batch* b = nullptr;
-            int st = bt_batch_upload(ctx_small, gs.data(), gs.size(), &b);
-            if (st == BT_OK) st = bt_batch_run(ctx_small, b, rs.front()->algo, outs.data(), res.data());
+            int st = bt_batch_upload(ctx, gs.data(), gs.size(), &b);
+            if (st == BT_OK) st = bt_batch_run(ctx, b, rs.front()->algo, outs.data(), res.data());
             if (b) bt_batch_free(b);
             if (st == BT_OK)
                 for (size_t i = 0; i < rs.size(); i++) rs[i]->res = res[i];
@@ -149,21 +166,8 @@ struct bt_server {
     }
 
     void run_large() {
-        for (;;) {
-            std::vector<RequestPtr> rs;
-            {
-                std::unique_lock<std::mutex> lk(mu);
-                for (;;) {
-                    if (!large_q.empty()) {
-                        rs = take(large_q, false);
-                        if (!rs.empty()) break;
-                    } else if (stopping) {
-                        return;
-                    }
-                    cv_work.wait(lk);
-                }
-                stats.large_runs++;
-            }
+        std::vector<RequestPtr> rs;
+        while (next(large_q, false, rs)) {
             Request& r = *rs.front();
             bt_dgraph* dg = nullptr;
             int st = bt_graph_upload(ctx_large, &r.g, &dg);
@@ -194,15 +198,31 @@ int bt_serve_start(const bt_serve_config* cfg, bt_server** out) {
     if (!s->cfg.budget_bytes) s->cfg.budget_bytes = 64ull << 30;
     if (!s->cfg.large_edges) s->cfg.large_edges = 1000000;
     if (!s->cfg.max_batch) s->cfg.max_batch = 1024;
-    int st = bt_ctx_create(s->cfg.device, &s->ctx_small);
+    if (!s->cfg.light_wedges) s->cfg.light_wedges = 2000000;
+    int st = bt_ctx_create(s->cfg.device, &s->ctx_light);
+    if (st == BT_OK) st = bt_ctx_create(s->cfg.device, &s->ctx_heavy);
     if (st == BT_OK) st = bt_ctx_create(s->cfg.device, &s->ctx_large);
+    if (st == BT_OK) {
+        // the light lane's peel on a quarter of the CUs, the others on the rest
+        int const cus = s->ctx_light->cu_count;
+        int const light = std::max(1, cus / 4);
+        st = bt_ctx_set_option(s->ctx_light, BT_OPT_PEEL_BLOCKS, light);
+        if (st == BT_OK) st = bt_ctx_set_option(s->ctx_heavy, BT_OPT_PEEL_BLOCKS, cus - light);
+        if (st == BT_OK) st = bt_ctx_set_option(s->ctx_large, BT_OPT_PEEL_BLOCKS, cus - light);
+    }
+    auto destroy_ctx = [s] {
+        if (s->ctx_light) bt_ctx_destroy(s->ctx_light);
+        if (s->ctx_heavy) bt_ctx_destroy(s->ctx_heavy);
+        if (s->ctx_large) bt_ctx_destroy(s->ctx_large);
+    };
     if (st != BT_OK) {
-        if (s->ctx_small) bt_ctx_destroy(s->ctx_small);
+        destroy_ctx();
         delete s;
         return st;
     }
     try {
-        s->thr_small = std::thread([s] { s->run_small(); });
+        s->thr_light = std::thread([s] { s->run_batches(s->light_q, s->ctx_light); });
+        s->thr_heavy = std::thread([s] { s->run_batches(s->heavy_q, s->ctx_heavy); });
         s->thr_large = std::thread([s] { s->run_large(); });
     } catch (...) {
         {
@@ -210,9 +230,9 @@ int bt_serve_start(const bt_serve_config* cfg, bt_server** out) {
             s->stopping = true;
         }
         s->cv_work.notify_all();
-        if (s->thr_small.joinable()) s->thr_small.join();
-        bt_ctx_destroy(s->ctx_small);
-        bt_ctx_destroy(s->ctx_large);
+        for (std::thread* t : {&s->thr_light, &s->thr_heavy, &s->thr_large})
+            if (t->joinable()) t->join();
+        destroy_ctx();
         delete s;
         return BT_ERR_NOMEM;
     }
@@ -248,14 +268,17 @@ int bt_serve_submit(bt_server* s, const bt_host_graph* g, int algo, uint64_t* ti
     }
     r->algo = algo;
     r->est_bytes = estimate_bytes(*g, wedges);
-    if (r->est_bytes > s->cfg.budget_bytes) return BT_ERR_BUDGET;
+    if (r->est_bytes + kRunBytes > s->cfg.budget_bytes) return BT_ERR_BUDGET;
     r->t_submit = Clock::now();
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->stopping) return BT_ERR_STATE;
         r->ticket = s->next_ticket++;
         s->all[r->ticket] = r;
-        (g->m >= s->cfg.large_edges ? s->large_q : s->small_q).push_back(r);
+        std::deque<RequestPtr>& q = g->m >= s->cfg.large_edges ? s->large_q
+                                    : wedges < s->cfg.light_wedges ? s->light_q
+                                                                   : s->heavy_q;
+        q.push_back(r);
         s->stats.submitted++;
         *ticket = r->ticket;
     }
@@ -305,9 +328,10 @@ int bt_serve_stop(bt_server* s) {
         s->stopping = true;
     }
     s->cv_work.notify_all();
-    if (s->thr_small.joinable()) s->thr_small.join();
-    if (s->thr_large.joinable()) s->thr_large.join();
-    bt_ctx_destroy(s->ctx_small);
+    for (std::thread* t : {&s->thr_light, &s->thr_heavy, &s->thr_large})
+        if (t->joinable()) t->join();
+    bt_ctx_destroy(s->ctx_light);
+    bt_ctx_destroy(s->ctx_heavy);
     bt_ctx_destroy(s->ctx_large);
     delete s;
     return BT_OK;

@@ -4,12 +4,33 @@ on top of the C-ABI in include/bitruss_hip.h.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 
 import numpy as np
 
 from . import _lib
 from .graph import Graph
+
+# Live handles, released in dependency order when the interpreter exits:
+# at shutdown module globals (this module's `_lib` among them) may be cleared
+# before the objects' __del__ runs, which would leave device memory, streams
+# and server threads to the HIP runtime's own teardown.
+_live: "weakref.WeakSet" = weakref.WeakSet()
+
+
+def _release_all() -> None:
+    for kind, method in (("Server", "stop"), ("Batch", "free"), ("DeviceGraph", "free"), ("Device", "close")):
+        for obj in list(_live):
+            if type(obj).__name__ == kind:
+                try:
+                    getattr(obj, method)()
+                except Exception:
+                    pass
+
+
+atexit.register(_release_all)
 
 
 def device_count() -> int:
@@ -25,6 +46,7 @@ class Device:
         self._ctx = C.c_void_p()
         _lib.check("bt_ctx_create", _lib.lib().bt_ctx_create(device, C.byref(self._ctx)))
         self.device = device
+        _live.add(self)
 
     @property
     def handle(self) -> C.c_void_p:
@@ -45,6 +67,10 @@ class Device:
         """"union" (default) or "per_graph": how msp walks a batch union's levels."""
         _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(
             self.handle, _lib.OPT_BATCH_LEVELS, _lib.BATCH_LEVELS[mode]))
+
+    def set_peel_blocks(self, cus: int) -> None:
+        """CUs the persistent msp peel occupies (0: all)."""
+        _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(self.handle, _lib.OPT_PEEL_BLOCKS, cus))
 
     def set_index_mem_pct(self, pct: int) -> None:
         _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(self.handle, _lib.OPT_INDEX_MEM_PCT, pct))
@@ -87,6 +113,7 @@ class DeviceGraph:
         self._h = C.c_void_p()
         hg = g.host_struct()
         _lib.check("bt_graph_upload", _lib.lib().bt_graph_upload(dev.handle, C.byref(hg), C.byref(self._h)))
+        _live.add(self)
 
     @property
     def handle(self) -> C.c_void_p:
@@ -150,6 +177,7 @@ class Batch:
         self._keep = graphs  # host arrays stay alive during the upload
         _lib.check("bt_batch_upload", _lib.lib().bt_batch_upload(dev.handle, hs, len(graphs), C.byref(self._h)))
         self._keep = None
+        _live.add(self)
 
     @property
     def handle(self) -> C.c_void_p:
@@ -200,11 +228,13 @@ class Server:
     decomposed together in batches, large ones one at a time, under a
     device-memory admission budget."""
 
-    def __init__(self, device: int = 0, budget_bytes: int = 0, large_edges: int = 0, max_batch: int = 0):
-        cfg = _lib.BtServeConfig(device, budget_bytes, large_edges, max_batch)
+    def __init__(self, device: int = 0, budget_bytes: int = 0, large_edges: int = 0, max_batch: int = 0,
+                 light_wedges: int = 0):
+        cfg = _lib.BtServeConfig(device, budget_bytes, large_edges, max_batch, light_wedges)
         self._h = C.c_void_p()
         _lib.check("bt_serve_start", _lib.lib().bt_serve_start(C.byref(cfg), C.byref(self._h)))
         self._m = {}
+        _live.add(self)
 
     def submit(self, g: Graph, algo: str = "msp") -> int:
         t = C.c_uint64()

@@ -92,8 +92,13 @@ int bt_ctx_last_timings(const bt_ctx* ctx, bt_timings* out);
  *                          have); BT_BATCH_LEVELS_PER_GRAPH: in each step
  *                          every graph peels its own next level (more phases
  *                          of less work; slower on the measured mixes).
- *                          Results are the same; only the speed differs. */
-enum { BT_OPT_MSP_PATH = 1, BT_OPT_INDEX_MEM_PCT = 2, BT_OPT_BATCH_WEDGES = 3, BT_OPT_BATCH_LEVELS = 4 };
+ *                          Results are the same; only the speed differs.
+ *   BT_OPT_PEEL_BLOCKS     CUs (one workgroup each) the persistent msp peel
+ *                          occupies (0, the default: all). Peels of several
+ *                          contexts run side by side while their CU counts
+ *                          fit the device; otherwise they wait for CUs. */
+enum { BT_OPT_MSP_PATH = 1, BT_OPT_INDEX_MEM_PCT = 2, BT_OPT_BATCH_WEDGES = 3, BT_OPT_BATCH_LEVELS = 4,
+       BT_OPT_PEEL_BLOCKS = 5 };
 enum { BT_MSP_PATH_AUTO = 0, BT_MSP_PATH_MERGE = 1, BT_MSP_PATH_INDEX = 2 };
 enum { BT_BATCH_LEVELS_PER_GRAPH = 0, BT_BATCH_LEVELS_UNION = 1 };
 int bt_ctx_set_option(bt_ctx* ctx, int option, int64_t value);
@@ -168,17 +173,19 @@ int bt_batch_run(bt_ctx* ctx, bt_batch* b, int algo, uint64_t* const* outs, bt_b
 
 /* ---- serving: a request loop with continuous batching ------------------
  *
- * No counterpart in the reference. A server owns two contexts on one device
- * and two worker threads. Small requests (fewer than large_edges edges) are
- * decomposed together: the batch worker takes every pending small request
- * of the oldest request's algorithm (up to max_batch) and runs them as one
- * batch, while requests that arrive meanwhile wait for the next batch.
- * Large requests run one at a time on the plain path in the other worker;
- * the two workers' non-persistent kernels overlap on the device. A request
- * is admitted when the estimated device bytes of the running requests plus
- * its own fit budget_bytes (the graph, ~140 B per edge, and its bloom
- * index, ~48 B per priority wedge); a request that alone exceeds the budget
- * is refused with BT_ERR_BUDGET. Results are those of the plain calls. */
+ * No counterpart in the reference. A server owns three lanes on one device,
+ * each a context and a worker thread. Small requests (fewer than
+ * large_edges edges) are decomposed together: a lane's worker takes every
+ * pending request of the oldest request's algorithm (up to max_batch) and
+ * runs them as one batch, while requests that arrive meanwhile wait for the
+ * next batch. Small requests with fewer than light_wedges priority wedges go
+ * to the light lane, whose persistent peel occupies a quarter of the CUs;
+ * the others to the heavy lane (the other CUs). Large requests run one at a
+ * time on the plain path, sharing the heavy lane's CUs. A request is
+ * admitted when the estimated device bytes of the running requests plus its
+ * own fit budget_bytes (the graph, ~140 B per edge, and its bloom index,
+ * ~48 B per priority wedge); a request that alone exceeds the budget is
+ * refused with BT_ERR_BUDGET. Results are those of the plain calls. */
 typedef struct bt_server bt_server;
 
 typedef struct bt_serve_config {
@@ -187,6 +194,8 @@ typedef struct bt_serve_config {
     uint32_t large_edges;   /* edges from which a request runs alone
                                (0: 1,000,000)                               */
     uint32_t max_batch;     /* requests per batch at most (0: 1024)         */
+    uint64_t light_wedges;  /* priority wedges below which a small request
+                               takes the light lane (0: 2,000,000)          */
 } bt_serve_config;
 
 typedef struct bt_serve_stats {

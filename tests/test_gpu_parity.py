@@ -193,3 +193,43 @@ def test_powerlaw_2m_golden_hidx(dev):
     assert int(h.max()) == gold["hidx_kmax"]
     assert rounds == gold["hidx_rounds_jacobi"]
     dg.free()
+
+
+@pytest.mark.gpu
+def test_peel_on_part_of_the_device(dev):
+    """The persistent peel on a subset of the CUs gives the same results,
+    also with two contexts peeling side by side on disjoint CU counts."""
+    import threading
+
+    gs = [gb.Graph.generate("mix_graph", i) for i in (5, 6, 2)]
+    want = []
+    for g in gs:
+        dg = dev.upload(g)
+        want.append(dg.msp())
+        dg.free()
+    ndev = dev.device
+    a, b = gb.Device(ndev), gb.Device(ndev)
+    a.set_peel_blocks(64)
+    b.set_peel_blocks(128)
+    got = {}
+
+    def run(d, name, graphs):
+        out = []
+        for g in graphs:
+            dg = d.upload(g)
+            out.append(dg.msp())
+            dg.free()
+        got[name] = out
+
+    ta = threading.Thread(target=run, args=(a, "a", gs))
+    tb = threading.Thread(target=run, args=(b, "b", gs[::-1]))
+    ta.start()
+    tb.start()
+    ta.join()
+    tb.join()
+    a.close()
+    b.close()
+    for (r, lv), (wr, wl) in zip(got["a"], want):
+        assert np.array_equal(r, wr) and lv == wl
+    for (r, lv), (wr, wl) in zip(got["b"], want[::-1]):
+        assert np.array_equal(r, wr) and lv == wl
