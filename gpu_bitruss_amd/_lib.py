@@ -58,6 +58,7 @@ class BtTimings(C.Structure):
         ("wedges_peel", C.c_uint64),
         ("sub_rounds", C.c_uint64),
         ("levels_scanned", C.c_uint64),
+        ("sorted_blooms", C.c_uint64),
     ]
 
 
@@ -106,6 +107,8 @@ OPT_BATCH_WEDGES = 3
 OPT_BATCH_LEVELS = 4
 BATCH_LEVELS = {"per_graph": 0, "union": 1}
 OPT_PEEL_BLOCKS = 5
+OPT_PEEL_SLICE_MS = 6
+OPT_BATCH_LANES = 7
 MSP_PATHS = {"auto": 0, "merge": 1, "index": 2}
 ALGOS = {"bfc": 1, "ebfc": 2, "hidx": 3, "msp": 4}
 

@@ -14,14 +14,21 @@
 // One union means one launch sequence for all its graphs: the counting
 // buckets the roots of all graphs together by wedge count, and msp runs
 // every graph's sub-rounds inside the same persistent launch, so the
-// latency of a phase is paid once for all of them. A batch whose bloom
-// records would not fit one index (32-bit record indices, the memory budget)
-// is split into consecutive unions, run one after the other.
+// latency of a phase is paid once for all of them. The graphs are spread
+// over lanes (contexts with their own stream and a share of the CUs) by
+// wedge count, largest first onto the lightest lane, and the lanes run side
+// by side: a phase of a union is short and bound by its slowest wave, so
+// several smaller unions on parts of the device finish sooner than one
+// large union on all of it. A lane whose bloom records would not fit one
+// index (32-bit record indices, the memory budget) holds several unions,
+// run one after the other.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstring>
 #include <new>
+#include <numeric>
+#include <thread>
 #include <vector>
 
 #include "../../include/bitruss_hip.h"
@@ -30,11 +37,12 @@
 
 using namespace bt;
 
-// One disjoint union of consecutive graphs [g0, g1) of a batch.
+// One disjoint union of graphs gid[0..] (batch indices, ascending) of a batch.
 struct bt_chunk {
     bt_dgraph* dg = nullptr;
-    size_t g0 = 0, g1 = 0;
-    std::vector<uint64_t> e_off;     // g1 - g0 + 1 edge offsets inside the union
+    std::vector<size_t> gid;
+    int lane = 0;
+    std::vector<uint64_t> e_off;     // gid.size() + 1 edge offsets inside the union
     uint64_t* d_eoff = nullptr;      // device copy
     uint32_t* egraph = nullptr;      // device, m: graph of each edge (chunk-local)
     uint32_t* glast = nullptr;       // device, per graph
@@ -45,8 +53,10 @@ struct bt_chunk {
 struct bt_batch {
     bt_ctx* ctx = nullptr;
     size_t k = 0;
+    int nlanes = 1;
     std::vector<bt_chunk> chunks;
-    std::vector<uint64_t> h_out;     // host staging of a union result
+    std::vector<bt_ctx*> lane_ctx;               // lanes beyond the caller's context (created on first run)
+    std::vector<std::vector<uint64_t>> h_out;    // per lane: host staging of a union result
 };
 
 namespace {
@@ -107,10 +117,12 @@ int chunk_alloc(bt_ctx* ctx, bt_chunk& c, T** p, size_t count) {
 }
 
 int chunk_upload(bt_ctx* ctx, const bt_host_graph* graphs, bt_chunk& c) {
-    size_t const k = c.g1 - c.g0;
+    size_t const k = c.gid.size();
+    std::vector<bt_host_graph> parts(k);
+    for (size_t i = 0; i < k; i++) parts[i] = graphs[c.gid[i]];
     bt_host_graph u{};
     c.e_off.assign(k + 1, 0);
-    int st = bt_graph_union(graphs + c.g0, k, &u, c.e_off.data());
+    int st = bt_graph_union(parts.data(), k, &u, c.e_off.data());
     if (st == BT_OK) st = bt_graph_upload(ctx, &u, &c.dg);
     bt_graph_release(&u);
     if (st != BT_OK) return st;
@@ -135,7 +147,7 @@ int chunk_run(bt_ctx* ctx, bt_chunk& c, int algo, uint64_t* const* outs, bt_batc
               std::vector<uint64_t>& h_out) {
     bt_dgraph* dg = c.dg;
     hipStream_t s = ctx->stream;
-    size_t const k = c.g1 - c.g0;
+    size_t const k = c.gid.size();
     uint32_t rounds = 0;
     const uint64_t* dres = nullptr;
     if (algo == BT_ALGO_BFC || algo == BT_ALGO_EBFC) {
@@ -170,12 +182,12 @@ int chunk_run(bt_ctx* ctx, bt_chunk& c, int algo, uint64_t* const* outs, bt_batc
     BT_HIP_TRY(hipStreamSynchronize(s));
     if (want_out) {
         for (size_t i = 0; i < k; i++)
-            if (outs[c.g0 + i] && c.e_off[i + 1] > c.e_off[i])
-                std::memcpy(outs[c.g0 + i], h_out.data() + c.e_off[i], sizeof(uint64_t) * (c.e_off[i + 1] - c.e_off[i]));
+            if (outs[c.gid[i]] && c.e_off[i + 1] > c.e_off[i])
+                std::memcpy(outs[c.gid[i]], h_out.data() + c.e_off[i], sizeof(uint64_t) * (c.e_off[i + 1] - c.e_off[i]));
     }
     if (results) {
         for (size_t i = 0; i < k; i++) {
-            bt_batch_result& r = results[c.g0 + i];
+            bt_batch_result& r = results[c.gid[i]];
             std::memset(&r, 0, sizeof(r));
             uint64_t const sum = red[2 * i], mx = red[2 * i + 1];
             if (algo == BT_ALGO_BFC || algo == BT_ALGO_EBFC) {
@@ -201,6 +213,7 @@ int bt_batch_free(bt_batch* b) {
     if (!b) return BT_ERR_INVALID;
     if (b->ctx) (void)hipSetDevice(b->ctx->device);
     for (bt_chunk& c : b->chunks) chunk_free(c, b->ctx);
+    for (bt_ctx* lc : b->lane_ctx) bt_ctx_destroy(lc);
     delete b;
     return BT_OK;
 }
@@ -215,36 +228,60 @@ int bt_batch_upload(bt_ctx* ctx, const bt_host_graph* graphs, size_t k, bt_batch
     if (!b) return BT_ERR_NOMEM;
     b->ctx = ctx;
     b->k = k;
-    // Consecutive graphs share a union while their priority wedges (which
-    // bound the bloom records) stay within what one bloom index may hold
-    // (record indices are 32-bit; about 44 bytes per record of the memory
-    // budget, plus the resident unions).
+    // Lanes: graphs by priority wedges (which bound the bloom records),
+    // largest first onto the lane with the fewest so far.
     std::vector<uint64_t> wedges(k, 0);
     for (size_t i = 0; i < k; i++) BT_TRY(bt_graph_wedges(&graphs[i], &wedges[i]));
+    int const nl = int(std::max<size_t>(1, std::min<size_t>(size_t(std::max(ctx->batch_lanes, 1)), k)));
+    b->nlanes = nl;
+    std::vector<std::vector<size_t>> lane_g(nl);
+    {
+        std::vector<size_t> order(k);
+        std::iota(order.begin(), order.end(), size_t(0));
+        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return wedges[x] > wedges[y]; });
+        std::vector<uint64_t> load(nl, 0);
+        for (size_t i : order) {
+            int const l = int(std::min_element(load.begin(), load.end()) - load.begin());
+            lane_g[l].push_back(i);
+            load[l] += wedges[i] + graphs[i].m;
+        }
+        for (auto& v : lane_g) std::sort(v.begin(), v.end());
+    }
+    // A lane's graphs share a union while their wedges stay within what one
+    // bloom index may hold: record indices are 32-bit, and the indexes of
+    // the lanes' unions are built side by side, at most 52 bytes per wedge
+    // (16 B emitted, 36 B sorted and indexed per record) within 85% of the
+    // free device memory after the unions' graphs (about 140 B per edge).
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
         delete b;
         return BT_ERR_DEVICE;
     }
-    uint64_t const by_mem = uint64_t(double(free_b) * ctx->index_mem_frac / 88.0);  // unions stay resident too
+    uint64_t m_all = 0;
+    for (size_t i = 0; i < k; i++) m_all += graphs[i].m;
+    double const room = std::max(0.0, double(free_b) * 0.85 - 140.0 * double(m_all));
+    uint64_t const by_mem = uint64_t(room / 52.0 / nl);
     uint64_t limit = std::max<uint64_t>(1, std::min<uint64_t>((1ull << 31) - 1, by_mem));
     if (ctx->batch_wedges) limit = std::min(limit, ctx->batch_wedges);
-    uint64_t acc = 0, acc_m = 0;
-    size_t g0 = 0;
-    for (size_t i = 0; i <= k; i++) {
-        uint64_t const w = i < k ? wedges[i] : 0;
-        uint64_t const gm = i < k ? graphs[i].m : 0;
-        bool const close = i == k || (i > g0 && (acc + w > limit || 2 * (acc_m + gm) >= (1ull << 32)));
-        if (close && i > g0) {
-            bt_chunk c;
-            c.g0 = g0;
-            c.g1 = i;
-            b->chunks.push_back(c);
-            g0 = i;
-            acc = acc_m = 0;
+    for (int l = 0; l < nl; l++) {
+        std::vector<size_t> const& gl = lane_g[l];
+        uint64_t acc = 0, acc_m = 0;
+        size_t i0 = 0;
+        for (size_t i = 0; i <= gl.size(); i++) {
+            uint64_t const w = i < gl.size() ? wedges[gl[i]] : 0;
+            uint64_t const gm = i < gl.size() ? graphs[gl[i]].m : 0;
+            bool const close = i == gl.size() || (i > i0 && (acc + w > limit || 2 * (acc_m + gm) >= (1ull << 32)));
+            if (close && i > i0) {
+                bt_chunk c;
+                c.gid.assign(gl.begin() + i0, gl.begin() + i);
+                c.lane = l;
+                b->chunks.push_back(std::move(c));
+                i0 = i;
+                acc = acc_m = 0;
+            }
+            acc += w;
+            acc_m += gm;
         }
-        acc += w;
-        acc_m += gm;
     }
     for (bt_chunk& c : b->chunks) {
         int const st = chunk_upload(ctx, graphs, c);
@@ -261,7 +298,53 @@ int bt_batch_run(bt_ctx* ctx, bt_batch* b, int algo, uint64_t* const* outs, bt_b
     if (!ctx || !b || b->ctx != ctx) return BT_ERR_INVALID;
     if (algo < BT_ALGO_BFC || algo > BT_ALGO_MSP) return BT_ERR_INVALID;
     BT_HIP_TRY(hipSetDevice(ctx->device));
-    for (bt_chunk& c : b->chunks) BT_TRY(chunk_run(ctx, c, algo, outs, results, b->h_out));
+    int const nl = b->nlanes;
+    b->h_out.resize(nl);
+    if (nl == 1) {
+        for (bt_chunk& c : b->chunks) BT_TRY(chunk_run(ctx, c, algo, outs, results, b->h_out[0]));
+        return BT_OK;
+    }
+    // lane contexts: each persistent peel takes its share of the CUs
+    int const cus = ctx->peel_blocks > 0 && ctx->peel_blocks < ctx->cu_count ? ctx->peel_blocks : ctx->cu_count;
+    int const share = std::max(1, cus / nl);
+    if (b->lane_ctx.empty()) {
+        for (int l = 0; l < nl; l++) {
+            bt_ctx* lc = nullptr;
+            int const st = bt_ctx_create(ctx->device, &lc);
+            if (st != BT_OK) return st;
+            b->lane_ctx.push_back(lc);
+        }
+    }
+    for (bt_ctx* lc : b->lane_ctx) {
+        lc->msp_path = ctx->msp_path;
+        lc->index_mem_frac = ctx->index_mem_frac;
+        lc->batch_levels = ctx->batch_levels;
+        lc->peel_slice_ms = ctx->peel_slice_ms;
+        lc->peel_blocks = share < lc->cu_count ? share : 0;
+    }
+    std::vector<int> status(nl, BT_OK);
+    auto run_lane = [&](int l) {
+        bt_ctx* lc = b->lane_ctx[l];
+        if (hipSetDevice(lc->device) != hipSuccess) {
+            status[l] = BT_ERR_DEVICE;
+            return;
+        }
+        for (bt_chunk& c : b->chunks) {
+            if (c.lane != l) continue;
+            c.dg->ctx = lc;  // the union runs on the lane's stream and index
+            int const st = chunk_run(lc, c, algo, outs, results, b->h_out[l]);
+            c.dg->ctx = ctx;
+            if (st != BT_OK) {
+                status[l] = st;
+                return;
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int l = 1; l < nl; l++) th.emplace_back(run_lane, l);
+    run_lane(0);
+    for (auto& t : th) t.join();
+    for (int st : status) BT_TRY(st);
     return BT_OK;
 }
 

@@ -420,16 +420,15 @@ int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total_out, uint64_t* s
     Emit em;
     c->ix.R = kNoIndex;
     if (emit && per_edge) {
-        // at most one record per priority wedge; the index (records, sorted
-        // copy, bloom ids, incidences) takes about 44 bytes per wedge
+        // at most one record per priority wedge, 16 bytes each in emission
+        // order: these buffers may take index_mem_frac of the free device
+        // memory; the rest of the index is sized once the records are counted
         bt_index& ix = c->ix;
         size_t const cap = std::max<uint64_t>(cnts[3], 1);
         size_t free_b = 0, total_b = 0;
         BT_HIP_TRY(hipMemGetInfo(&free_b, &total_b));
-        size_t const have = ix.keys0.bytes + ix.keys1.bytes + ix.vals0.bytes + ix.vals1.bytes + ix.bid.bytes +
-                            ix.inc.bytes;
-        if (double(cap) * 44.0 > (double(free_b) + double(have)) * c->index_mem_frac || cap >= (1ull << 31))
-            emit = false;
+        size_t const have = ix.keys0.bytes + ix.vals0.bytes;
+        if (double(cap) * 16.0 > (double(free_b) + double(have)) * c->index_mem_frac) emit = false;
     }
     if (emit && per_edge) {
         bt_index& ix = c->ix;
@@ -510,6 +509,19 @@ int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total_out, uint64_t* s
     if (total_out) *total_out = res[0];
     if (smax_out) *smax_out = res[1];
     c->ix.R = em.keys ? res[2] : kNoIndex;
+    if (em.keys) {
+        // the rest of the index, per record: sorted keys and values (16 B),
+        // bloom ids, sizes and claim stamps (12 B) and incidences (8 B); it
+        // must fit the free device memory with a tenth to spare, and records
+        // are named by u32 indices
+        bt_index& ix = c->ix;
+        size_t free_b = 0, total_b = 0;
+        BT_HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+        size_t const have = ix.keys1.bytes + ix.vals1.bytes + ix.bid.bytes + ix.bk.bytes + ix.bclaim.bytes +
+                            ix.inc.bytes;
+        double const need = double(res[2]) * 36.0;
+        if (res[2] > kMaxIndexRecords || need > (double(free_b) + double(have)) * 0.9) c->ix.R = kNoIndex;
+    }
     dg->has_support = per_edge;
     c->last.prep_ms = tm.ms(0, 1);
     c->last.count_kernel_ms = tm.ms(2, 3);

@@ -39,7 +39,7 @@ constexpr uint32_t kHB = 1024;        // histogram bins per wave
 
 __global__ void k_hidx_mv(uint32_t R, const uint64_t* __restrict__ rec, const uint64_t* __restrict__ prev,
                           uint64_t* __restrict__ mv) {
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < R; p += gridDim.x * blockDim.x) {
+    for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < R; p += uint64_t(gridDim.x) * blockDim.x) {
         uint64_t const r = rec[p];
         uint64_t const a = prev[uint32_t(r >> 32)], b = prev[uint32_t(r)];
         mv[p] = a < b ? a : b;
@@ -50,7 +50,7 @@ struct HidxArgs {
     uint32_t m;
     const uint64_t* prev;
     uint64_t* curr;
-    const uint32_t* inc_off;
+    const uint64_t* inc_off;
     const uint32_t* inc;     // per incidence: record (middle) holding the edge
     const uint64_t* rec;     // middles: id(r-v) << 32 | id(v-w)
     const uint32_t* bid;     // per record: bloom index + 1
@@ -66,12 +66,12 @@ struct HidxArgs {
 // (= prev[e], which bounds the answer) and bins those in [lo, hi] into
 // hist[(v - lo) / width]; values above hi are the ones already counted
 // above the range.
-__device__ __forceinline__ void bin_values(const HidxArgs& a, uint32_t e, uint32_t q0, uint32_t q1, uint64_t lo,
+__device__ __forceinline__ void bin_values(const HidxArgs& a, uint32_t e, uint64_t q0, uint64_t q1, uint64_t lo,
                                            uint64_t hi, uint64_t width, uint64_t cap, uint32_t* hist, uint32_t* s_incl,
                                            uint32_t* s_base, uint32_t* s_skip, uint64_t* s_pp) {
     int const lane = threadIdx.x & (kWave - 1);
-    for (uint32_t b0 = q0; b0 < q1; b0 += kWave) {
-        uint32_t const q = b0 + lane;
+    for (uint64_t b0 = q0; b0 < q1; b0 += kWave) {
+        uint64_t const q = b0 + lane;
         uint32_t len = 0, base = 0, skip = 0;
         uint64_t pp = 0;
         if (q < q1) {
@@ -93,7 +93,7 @@ __device__ __forceinline__ void bin_values(const HidxArgs& a, uint32_t e, uint32
         s_skip[lane] = skip;
         s_pp[lane] = pp;
         __builtin_amdgcn_wave_barrier();
-        uint32_t const nb = b0 + kWave <= q1 ? uint32_t(kWave) : q1 - b0;
+        uint32_t const nb = b0 + kWave <= q1 ? uint32_t(kWave) : uint32_t(q1 - b0);
         for (uint32_t t0 = 0; t0 < total; t0 += kWave) {
             uint32_t const t = t0 + lane;
             if (t < total) {
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(kHT) void k_hidx_round(HidxArgs a) {
             if (lane == 0) a.curr[e] = 0;
             continue;
         }
-        uint32_t const q0 = a.inc_off[e], q1 = a.inc_off[e + 1];
+        uint64_t const q0 = a.inc_off[e], q1 = a.inc_off[e + 1];
         // candidates [lo, hi]; `above` values lie above hi
         uint64_t lo = 0, hi = pe, above = 0;
         uint64_t width = (pe + kHB) / kHB;  // ceil((pe + 1) / kHB)
@@ -230,7 +230,7 @@ int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds_out) {
             HidxArgs a{m,
                        prev,
                        curr,
-                       static_cast<const uint32_t*>(ix.inc_off.ptr),
+                       static_cast<const uint64_t*>(ix.inc_off.ptr),
                        static_cast<const uint32_t*>(ix.inc.ptr),
                        static_cast<const uint64_t*>(ix.vals1.ptr),
                        static_cast<const uint32_t*>(ix.bid.ptr),

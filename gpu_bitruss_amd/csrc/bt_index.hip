@@ -17,7 +17,7 @@
 //   bclaim[R + ...]  at s: claim stamp of the peel (0 here); then the
 //                    peel's dummy stamps (kClaimDummiesPerBlock per block)
 //   bgraph[R]        batches only, at s: the graph of the bloom
-//   inc_off[m+1]     per-edge offsets into inc
+//   inc_off[m+1] (u64) per-edge offsets into inc
 //   inc[2R]          first records of the blooms holding the edge (or, for
 //                    the h-index rounds, the records holding it)
 #include <hip/hip_runtime.h>
@@ -49,7 +49,7 @@ __global__ void k_bloom_size(uint32_t nbloom, uint32_t R, int nb, const uint64_t
                              const uint8_t* __restrict__ r_up, uint32_t* __restrict__ bstart,
                              uint32_t* __restrict__ bk, uint32_t* __restrict__ bclaim,
                              const uint64_t* __restrict__ rec, const uint32_t* __restrict__ egraph,
-                             uint32_t* __restrict__ bgraph) {
+                             uint32_t* __restrict__ bgraph, uint32_t* __restrict__ kmax) {
     uint32_t const b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b == 0) bstart[nbloom] = R;
     if (b >= nbloom) return;
@@ -57,6 +57,7 @@ __global__ void k_bloom_size(uint32_t nbloom, uint32_t R, int nb, const uint64_t
     uint32_t const e = (b + 1 < nbloom) ? bstart[b + 1] : R;
     uint32_t const r = uint32_t(keys[s] >> nb);
     bk[s] = (e - s) | (uint32_t(r_up[r]) << 31);
+    atomicMax(kmax, e - s);
     bclaim[s] = 0;
     if (egraph) bgraph[s] = egraph[uint32_t(rec[s] >> 32)];  // batches: the bloom's graph
 }
@@ -70,7 +71,7 @@ __global__ void k_inc_count(uint64_t R, const uint64_t* __restrict__ rec, uint32
 }
 
 __global__ void k_inc_fill(uint64_t R, const uint64_t* __restrict__ rec, const uint32_t* __restrict__ bid,
-                           const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ inc_off,
+                           const uint32_t* __restrict__ bstart, const uint64_t* __restrict__ inc_off,
                            uint32_t* __restrict__ cur, uint32_t* __restrict__ inc, bool records) {
     for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < R; i += uint64_t(gridDim.x) * blockDim.x) {
         uint64_t const v = rec[i];
@@ -95,16 +96,17 @@ int bt_index_build(bt_dgraph* dg, bool inc_records) {
     hipStream_t s = c->stream;
     uint32_t const m = dg->m;
     uint64_t const R = ix.R;
-    if (R >= (1ull << 31)) return BT_ERR_OVERFLOW;
+    if (R > kMaxIndexRecords) return BT_ERR_OVERFLOW;
     constexpr unsigned T = 256;
     unsigned const grid = grid_for(R ? R : 1, T, unsigned(c->cu_count) * 16);
-    BT_TRY(bt_buf_reserve(ix.inc_off, sizeof(uint32_t) * (size_t(m) + 1)));
+    BT_TRY(bt_buf_reserve(ix.inc_off, sizeof(uint64_t) * (size_t(m) + 1)));
     BT_TRY(bt_buf_reserve(ix.inc_cur, sizeof(uint32_t) * (size_t(m) + 1)));
-    uint32_t* inc_off = static_cast<uint32_t*>(ix.inc_off.ptr);
+    uint64_t* inc_off = static_cast<uint64_t*>(ix.inc_off.ptr);
     uint32_t* cur = static_cast<uint32_t*>(ix.inc_cur.ptr);
     if (R == 0) {
         ix.NB = 0;
-        BT_HIP_TRY(hipMemsetAsync(inc_off, 0, sizeof(uint32_t) * (size_t(m) + 1), s));
+        BT_HIP_TRY(hipMemsetAsync(dg->ctl + kCtlBloomMax, 0, sizeof(uint64_t), s));
+        BT_HIP_TRY(hipMemsetAsync(inc_off, 0, sizeof(uint64_t) * (size_t(m) + 1), s));
         BT_TRY(bt_buf_reserve(ix.bstart, 64));
         BT_TRY(bt_buf_reserve(ix.bk, 64));
         BT_TRY(bt_buf_reserve(ix.bclaim, sizeof(uint32_t) * size_t(c->cu_count) * kClaimDummiesPerBlock));
@@ -126,7 +128,8 @@ int bt_index_build(bt_dgraph* dg, bool inc_records) {
     size_t t1 = 0, t2 = 0, t3 = 0;
     BT_HIP_TRY(rocprim::radix_sort_pairs(nullptr, t1, k0, k1, v0, v1, size_t(R), 0, 2 * nb, s));
     BT_HIP_TRY(rocprim::inclusive_scan(nullptr, t2, flag, bid, size_t(R), rocprim::plus<uint32_t>(), s));
-    BT_HIP_TRY(rocprim::exclusive_scan(nullptr, t3, cur, inc_off, 0u, size_t(m) + 1, rocprim::plus<uint32_t>(), s));
+    BT_HIP_TRY(rocprim::exclusive_scan(nullptr, t3, cur, inc_off, uint64_t(0), size_t(m) + 1, rocprim::plus<uint64_t>(),
+                                       s));
     BT_TRY(bt_buf_reserve(c->sort_tmp, std::max(t1, std::max(t2, t3))));
     BT_HIP_TRY(rocprim::radix_sort_pairs(c->sort_tmp.ptr, t1, k0, k1, v0, v1, size_t(R), 0, 2 * nb, s));
     k_heads<<<grid, T, 0, s>>>(R, k1, flag);
@@ -150,13 +153,15 @@ int bt_index_build(bt_dgraph* dg, bool inc_records) {
         BT_TRY(bt_buf_reserve(ix.bgraph, sizeof(uint32_t) * R));
         bgraph = static_cast<uint32_t*>(ix.bgraph.ptr);
     }
+    uint32_t* kmax = reinterpret_cast<uint32_t*>(dg->ctl + kCtlBloomMax);
+    BT_HIP_TRY(hipMemsetAsync(kmax, 0, sizeof(uint32_t), s));
     k_bloom_size<<<grid_for(nbloom, T, 1u << 30), T, 0, s>>>(nbloom, uint32_t(R), nb, k1, dg->r_up, bstart, bk, bclaim,
-                                                              v1, dg->egraph, bgraph);
+                                                              v1, dg->egraph, bgraph, kmax);
     // edge -> blooms incidence lists
     BT_HIP_TRY(hipMemsetAsync(cur, 0, sizeof(uint32_t) * (size_t(m) + 1), s));
     k_inc_count<<<grid, T, 0, s>>>(R, v1, cur);
-    BT_HIP_TRY(rocprim::exclusive_scan(c->sort_tmp.ptr, t3, cur, inc_off, 0u, size_t(m) + 1,
-                                       rocprim::plus<uint32_t>(), s));
+    BT_HIP_TRY(rocprim::exclusive_scan(c->sort_tmp.ptr, t3, cur, inc_off, uint64_t(0), size_t(m) + 1,
+                                       rocprim::plus<uint64_t>(), s));
     BT_HIP_TRY(hipMemsetAsync(cur, 0, sizeof(uint32_t) * (size_t(m) + 1), s));
     k_inc_fill<<<grid, T, 0, s>>>(R, v1, bid, bstart, inc_off, cur, inc, inc_records);
     BT_HIP_TRY(hipGetLastError());

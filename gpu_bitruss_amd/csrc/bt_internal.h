@@ -50,6 +50,13 @@ void bt_buf_free(bt_buf& b);
 // lanes with nothing to claim (one per thread and unroll slot of a block).
 constexpr uint32_t kClaimDummiesPerBlock = 2048;
 
+// Records and blooms are named by u32 record indices, and the claim stamps
+// of the peel's dummy lanes follow the records: the index holds at most
+// 2^32 - 1 - (dummies of the largest grid) records.
+constexpr uint64_t kMaxIndexRecords = 0xFFFFFFFFull - 1024ull * kClaimDummiesPerBlock;
+// dg->ctl word (u32) where the index build leaves its largest bloom size
+constexpr int kCtlBloomMax = 120;
+
 // Bloom index of a graph (wedges grouped by (root, far end) with >= 2
 // middles), rebuilt by each msp call; buffers are reused across calls.
 struct bt_index {
@@ -59,7 +66,7 @@ struct bt_index {
     bt_buf bstart;        // u32 per bloom: first record, plus a sentinel R
     bt_buf bk;            // u32 per record, used at bloom starts: live middles | root-is-upper << 31
     bt_buf bclaim;        // u32 per record, used at bloom starts: last claiming sub-round
-    bt_buf inc_off;       // u32 per edge + 1: incidence offsets
+    bt_buf inc_off;       // u64 per edge + 1: incidence offsets (2R may pass 2^32)
     bt_buf inc_cur;       // u32 per edge: fill cursors
     bt_buf inc;           // u32 per incidence: first record of the bloom
     bt_buf touched;       // u32 3 x blooms: shared claimed-bloom lists
@@ -68,6 +75,7 @@ struct bt_index {
     bt_buf glev;          // u64 2 x graphs: per-graph levels of the peel (batches)
     bt_buf live;          // u32 per edge: per-block live-edge segments
     bt_buf seg;           // u32 per block + 1: segment offsets, then counts
+    bt_buf c2s;           // u32 per peel wave: sorted both-frontier lists
     uint64_t R = 0, NB = 0;
 };
 
@@ -89,7 +97,9 @@ struct bt_ctx {
     double index_mem_frac = 0.5;   // share of free device memory the index may use
     uint64_t batch_wedges = 0;     // wedge bound per batch union (0: the index limits)
     int batch_levels = 1;          // BT_BATCH_LEVELS_*: how a union's msp walks levels
+    int batch_lanes = 2;           // batches: lanes (contexts on CU shares) run side by side
     int peel_blocks = 0;           // CUs the persistent peel uses (0: all)
+    int peel_slice_ms = 2000;      // time slice of a persistent peel launch (0: one launch)
     // device blocks given back by freed graphs and batches, reused by the
     // next ones (hipFree would wait for every stream on the device)
     std::mutex cache_mu;

@@ -49,6 +49,8 @@
 // memory operations (s_waitcnt 0). Read-only index data uses plain loads.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -87,6 +89,14 @@ struct MspCtl {
     unsigned long long t_collect, t_peel, t_work_collect, t_work_peel;
     unsigned long long pmax[2], pmax_sum;
     unsigned long long hist[24];
+    // time slices: block 0 sets `yield` before each peel barrier once the
+    // launch has run its slice; at the next level boundary every block sees
+    // it, and block 0 leaves the place to resume from (sub-round, level
+    // step, level, barrier epoch)
+    unsigned long long yield, yielded;
+    unsigned long long r_X, r_k, r_L, r_epoch;
+    unsigned long long t_launch;     // block 0's clock at the launch
+    unsigned long long sorted;       // bloom visits that sorted both-frontier lists
 };
 static_assert(sizeof(MspCtl) <= 128 * sizeof(uint64_t), "control block overlays dg->ctl");
 
@@ -98,7 +108,7 @@ struct MspArgs {
     uint64_t* sup;
     uint64_t* res;
     uint32_t* stamp;
-    const uint32_t* inc_off;
+    const uint64_t* inc_off;
     const uint32_t* inc;   // per incidence: first record of the bloom
     uint64_t* rec;         // middles, compacted in place
     uint32_t* hdr;         // at a bloom's first record: live middles | upper << 31
@@ -120,6 +130,17 @@ struct MspArgs {
     const uint32_t* bgraph;  // at a bloom's first record: its graph
     uint64_t* glev;          // [2][ngraph] per-graph level of the current / next step
     uint32_t* any;           // per block: live edges left
+    // time slices (plain graphs): a launch returns at the first level
+    // boundary after slice_ticks of the 100 MHz clock (0: never), and a
+    // launch with resume set continues from the place left in ctl
+    uint64_t slice_ticks;
+    uint32_t resume;
+    // per wave: scratch of 3 * (largest bloom) u32 for the sorted lists of
+    // both-frontier middles, and the bits of the largest edge id
+    uint32_t* c2s;
+    uint32_t c2s_stride;
+    int id_bits;
+    uint32_t c2cap;        // both-frontier middles summed from the LDS list (<= kC2Cap)
 };
 
 // relaxed device-scope accesses (coherent across XCDs)
@@ -279,7 +300,7 @@ struct Out {
 // up to 64 edges.
 struct WaveScan {
     uint32_t* incl;  // inclusive prefix of list lengths
-    uint32_t* off;   // incidence index minus flattened position
+    uint64_t* off;   // incidence index minus flattened position
 };
 
 // Claims, for sub-round Y, every bloom that holds one of the n edges in xl
@@ -294,11 +315,12 @@ __device__ __forceinline__ void claim_edges(const uint32_t* xl, uint32_t n, uint
     int const lane = threadIdx.x & (kWave - 1);
     for (uint32_t b0 = 0; b0 < n; b0 += kWave) {
         uint32_t const cnt = n - b0 < uint32_t(kWave) ? n - b0 : uint32_t(kWave);
-        uint32_t p0 = 0, len = 0;
+        uint64_t p0 = 0;
+        uint32_t len = 0;
         if (uint32_t(lane) < cnt) {
             uint32_t const e = xl[b0 + lane];
             p0 = a.inc_off[e];
-            len = a.inc_off[e + 1] - p0;
+            len = uint32_t(a.inc_off[e + 1] - p0);
         }
         uint32_t const incl = wave_incl_scan(len);
         uint32_t const total = __shfl(incl, kWave - 1, kWave);
@@ -360,7 +382,7 @@ struct CrossBuf {
 // waves' last pushes.
 __device__ __forceinline__ void claim_block(const MspArgs& a, uint32_t Y, const Out& out,
                                             uint32_t (*s_xl)[kXCap], const uint32_t* s_xn, uint32_t* s_incl,
-                                            uint32_t* s_off, uint32_t* s_part) {
+                                            uint64_t* s_off, uint32_t* s_part) {
     uint32_t const tid = threadIdx.x;
     uint32_t pre[kWPB + 1];
     pre[0] = 0;
@@ -370,7 +392,8 @@ __device__ __forceinline__ void claim_block(const MspArgs& a, uint32_t Y, const 
     // batches of kBL edges; thread t handles entries 2t and 2t + 1 of a batch
     for (uint32_t b0 = 0; b0 < n_all; b0 += kBL) {
         uint32_t const n = n_all - b0 < uint32_t(kBL) ? n_all - b0 : uint32_t(kBL);
-        uint32_t p0[2] = {0, 0}, len[2] = {0, 0};
+        uint64_t p0[2] = {0, 0};
+        uint32_t len[2] = {0, 0};
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             uint32_t const j = 2 * tid + h;
@@ -381,7 +404,7 @@ __device__ __forceinline__ void claim_block(const MspArgs& a, uint32_t Y, const 
                 for (int i = 1; i < kWPB; i++) w += jj >= pre[i] ? 1 : 0;
                 uint32_t const e = s_xl[w][jj - pre[w]];
                 p0[h] = a.inc_off[e];
-                len[h] = a.inc_off[e + 1] - p0[h];
+                len[h] = uint32_t(a.inc_off[e + 1] - p0[h]);
             }
         }
         uint32_t total = 0;
@@ -466,26 +489,141 @@ __device__ __forceinline__ uint64_t rule_sum(const uint64_t* c2, uint32_t n2, bo
     return d;
 }
 
-// Same sum read from the bloom in global memory (more both-frontier
-// middles than the LDS list holds).
-__device__ __forceinline__ uint64_t rule_scan(const MspArgs& a, const uint64_t* rec, uint32_t kb, uint32_t X, bool froot,
-                              uint32_t up, uint32_t g) {
-    uint64_t d = 0;
-    for (uint32_t t = 0; t < kb; t++) {
-        uint64_t const rr = ald(&rec[t]);
-        if (classify(ald(&a.stamp[uint32_t(rr >> 32)]), ald(&a.stamp[uint32_t(rr)]), X) != 2) continue;
-        uint32_t const o = froot ? uint32_t(rr) : uint32_t(rr >> 32);
-        uint32_t const sr = froot ? uint32_t(rr >> 32) : uint32_t(rr);
-        d += up ? rule3(o, sr, g) : rule3(o, g, sr);
+// Orders the wave's global-memory accesses to its scratch before the next
+// phase reads them from other lanes (the wave's CU holds the L1 they use).
+__device__ __forceinline__ void scratch_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Stable LSD radix sort of n u32 keys below 2^bits in global memory, by one
+// wave, 8 bits per pass with a 256-bin LDS histogram; tmp holds n keys.
+__device__ __forceinline__ void wave_radix_sort(uint32_t* keys, uint32_t n, uint32_t* tmp, uint32_t* hist, int bits) {
+    int const lane = threadIdx.x & (kWave - 1);
+    uint32_t* src = keys;
+    uint32_t* dst = tmp;
+    for (int sh = 0; sh < bits; sh += 8) {
+        for (int i = lane; i < 256; i += kWave) hist[i] = 0;
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t i = lane; i < n; i += kWave) atomicAdd(&hist[(src[i] >> sh) & 255u], 1u);
+        __builtin_amdgcn_wave_barrier();
+        // bins 4 * lane .. 4 * lane + 3 become their first output positions
+        uint32_t h[4], sum = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            h[j] = hist[4 * lane + j];
+            sum += h[j];
+        }
+        uint32_t pos = wave_incl_scan(sum) - sum;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            hist[4 * lane + j] = pos;
+            pos += h[j];
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t b = 0; b < n; b += kWave) {
+            uint32_t const i = b + lane;
+            bool const v = i < n;
+            uint32_t const key = v ? src[i] : 0u;
+            uint32_t const d = (key >> sh) & 255u;
+            // lanes with the same digit keep their order
+            uint64_t peers = __ballot(v);
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                uint64_t const m = __ballot((d >> t) & 1u);
+                peers &= ((d >> t) & 1u) ? m : ~m;
+            }
+            uint32_t const rank = __popcll(peers & lanemask_lt());
+            uint32_t const at = v ? hist[d] : 0u;
+            __builtin_amdgcn_wave_barrier();
+            if (v) dst[at + rank] = key;
+            if (v && rank == 0) hist[d] = at + uint32_t(__popcll(peers));
+            __builtin_amdgcn_wave_barrier();
+        }
+        scratch_fence();
+        uint32_t* const t = src;
+        src = dst;
+        dst = t;
     }
-    return d;
+    if (src != keys) {
+        for (uint32_t i = lane; i < n; i += kWave) keys[i] = src[i];
+        scratch_fence();
+    }
+}
+
+// Entries of the sorted list l[0, n) below g.
+__device__ __forceinline__ uint32_t count_below(const uint32_t* l, uint32_t n, uint32_t g) {
+    uint32_t pos = 0;
+    for (uint32_t step = n ? (1u << (31 - __builtin_clz(n))) : 0u; step; step >>= 1)
+        if (pos + step <= n && l[pos + step - 1] < g) pos += step;
+    return pos;
+}
+
+// The both-frontier middles of a bloom as four sorted id lists in the
+// wave's scratch: e1 and e2 of the middles with e1 < e2 (set P), then of
+// those with e2 < e1 (set Q). rule_sum over them is then a few counts of
+// entries below g (the middles of a bloom have distinct edges, so no entry
+// equals g):
+//   up,  froot:  |Q| - #(e1Q < g) + #(e2P < g)
+//   up,  !froot: |P| - #(e2P < g) + #(e1Q < g)
+//   !up, froot:  |Q| + #(e1Q < g) - #(e2Q < g) + #(e2P < g)
+//   !up, !froot: |P| + #(e2P < g) - #(e1P < g) + #(e1Q < g)
+// so a bloom with more both-frontier middles than the LDS list holds costs
+// a sort and a few binary searches per one-frontier middle, not a pass
+// over the bloom for each.
+struct C2Lists {
+    uint32_t *e1p, *e2p, *e1q, *e2q;
+    uint32_t np, nq;
+    uint32_t* tmp;  // sort buffer
+    uint32_t cp, cq;
+    __device__ __forceinline__ C2Lists(uint32_t* scratch, uint32_t np_, uint32_t nq_)
+        : e1p(scratch), e2p(scratch + np_), e1q(scratch + 2 * np_), e2q(scratch + 2 * np_ + nq_), np(np_), nq(nq_),
+          tmp(scratch + 2 * (np_ + nq_)), cp(0), cq(0) {}
+    // appends middle r of the lanes where c2 holds (whole wave)
+    __device__ __forceinline__ void put(bool c2, uint64_t r) {
+        uint32_t const e1 = uint32_t(r >> 32), e2 = uint32_t(r);
+        bool const p = c2 && e1 < e2, q = c2 && e2 < e1;
+        uint64_t const bp = __ballot(p), bq = __ballot(q);
+        if (p) {
+            uint32_t const k = cp + __popcll(bp & lanemask_lt());
+            e1p[k] = e1;
+            e2p[k] = e2;
+        }
+        if (q) {
+            uint32_t const k = cq + __popcll(bq & lanemask_lt());
+            e1q[k] = e1;
+            e2q[k] = e2;
+        }
+        cp += __popcll(bp);
+        cq += __popcll(bq);
+    }
+    // after the puts (whole wave)
+    __device__ __forceinline__ void sort(uint32_t* hist, int bits) {
+        scratch_fence();
+#pragma unroll 1
+        for (int i = 0; i < 4; i++)
+            wave_radix_sort(i == 0 ? e1p : i == 1 ? e2p : i == 2 ? e1q : e2q, i < 2 ? np : nq, tmp, hist, bits);
+    }
+    __device__ __forceinline__ uint64_t rule(bool froot, uint32_t up, uint32_t g) const {
+        if (up)
+            return froot ? uint64_t(nq - count_below(e1q, nq, g)) + count_below(e2p, np, g)
+                         : uint64_t(np - count_below(e2p, np, g)) + count_below(e1q, nq, g);
+        return froot ? uint64_t(nq + count_below(e1q, nq, g) - count_below(e2q, nq, g)) + count_below(e2p, np, g)
+                     : uint64_t(np + count_below(e2p, np, g) - count_below(e1p, np, g)) + count_below(e1q, nq, g);
+    }
+};
+
+// scratch of the calling wave for its C2Lists (3 entries per bloom middle)
+__device__ __forceinline__ uint32_t* wave_scratch(const MspArgs& a) {
+    uint32_t const w = __builtin_amdgcn_readfirstlane(blockIdx.x * kWPB + threadIdx.x / kWave);
+    return a.c2s + size_t(w) * a.c2s_stride;
 }
 
 // Processes a claimed bloom with kb <= 64 * RPT middles held in registers:
 // all middles and their stamps are loaded up front, so the bloom costs a
 // few dependent memory steps whatever its size. r0 holds middle `lane`.
 template <int RPT>
-__device__ __forceinline__ void process_bloom(const MspArgs& a, uint32_t s, uint32_t kb, uint32_t up, uint64_t r0,
+__device__ __forceinline__ bool process_bloom(const MspArgs& a, uint32_t s, uint32_t kb, uint32_t up, uint64_t r0,
                                               uint32_t X, Dec& dec, uint64_t* c2, CrossBuf& xb,
                                               const Out& st) {
     int const lane = threadIdx.x & (kWave - 1);
@@ -522,7 +660,9 @@ __device__ __forceinline__ void process_bloom(const MspArgs& a, uint32_t s, uint
     }
     __builtin_amdgcn_wave_barrier();
     uint32_t const dec0 = n1 + n2;
-    bool const c2ok = n2 <= uint32_t(kC2Cap);
+    // more both-frontier middles than the LDS list holds, and middles that
+    // need the rule: the streaming path (nothing is written yet)
+    if (n2 > a.c2cap && n1) return false;
     // decrements: the first target is r-v (c = 0) or the survivor (c = 1),
     // the second is v-w (c = 0 only, by dec0)
     uint32_t t1[RPT], d1[RPT];
@@ -538,7 +678,7 @@ __device__ __forceinline__ void process_bloom(const MspArgs& a, uint32_t s, uint
             uint32_t const g = froot ? e2 : e1;
             t1[i] = froot ? e1 : e2;
             d1[i] = n0 + (n1 - 1) +
-                    uint32_t(c2ok ? rule_sum(c2, n2, froot, up, g) : rule_scan(a, rec, kb, X, froot, up, g));
+                    uint32_t(rule_sum(c2, n2, froot, up, g));
         }
     }
     // the atomics of several rows (8, or 4 in the largest tier) are in
@@ -577,6 +717,7 @@ __device__ __forceinline__ void process_bloom(const MspArgs& a, uint32_t s, uint
             if (n0 < 2) afetch_max(&a.claim[s], ~0u);
         }
     }
+    return true;
 }
 
 // Blooms above 1024 middles: a counting pass, then a decrement pass, then
@@ -585,7 +726,7 @@ __device__ __forceinline__ void process_large(const MspArgs& a, uint32_t s, uint
                               uint64_t* c2, CrossBuf& xb, const Out& st) {
     int const lane = threadIdx.x & (kWave - 1);
     uint64_t* rec = a.rec + s;
-    uint32_t n0 = 0, n1 = 0, n2 = 0;
+    uint32_t n0 = 0, n1 = 0, n2 = 0, np = 0;
     for (uint32_t j0 = 0; j0 < kb; j0 += kWave) {
         uint32_t const j = j0 + lane;
         int c = -1;
@@ -602,10 +743,26 @@ __device__ __forceinline__ void process_large(const MspArgs& a, uint32_t s, uint
         n0 += __popcll(b0);
         n1 += __popcll(b1);
         n2 += __popcll(b2);
+        np += __popcll(__ballot(c == 2 && uint32_t(r >> 32) < uint32_t(r)));
     }
     __builtin_amdgcn_wave_barrier();
     uint64_t const dec0 = uint64_t(n1) + n2;
-    bool const inline_compact = n2 <= uint32_t(kC2Cap);
+    bool const c2ok = n2 <= a.c2cap;
+    C2Lists lists(wave_scratch(a), np, n2 - np);
+    if (!c2ok && n1) {
+        for (uint32_t j0 = 0; j0 < kb; j0 += kWave) {
+            uint32_t const j = j0 + lane;
+            int c = -1;
+            uint64_t r = 0;
+            if (j < kb) {
+                r = ald(&rec[j]);
+                c = classify(ald(&a.stamp[uint32_t(r >> 32)]), ald(&a.stamp[uint32_t(r)]), X);
+            }
+            lists.put(c == 2, r);
+        }
+        lists.sort(reinterpret_cast<uint32_t*>(c2), a.id_bits);
+        if (lane == 0) afetch_add(&a.ctl->sorted, 1ull);
+    }
     uint32_t wpos = 0;
     for (uint32_t j0 = 0; j0 < kb; j0 += kWave) {
         xb.ensure(2 * kWave, X + 1, a, st);
@@ -629,31 +786,16 @@ __device__ __forceinline__ void process_large(const MspArgs& a, uint32_t s, uint
             bool const froot = s1 != X;
             f = froot ? e1 : e2;
             uint32_t const g = froot ? e2 : e1;
-            uint64_t const d = uint64_t(n0) + (n1 - 1) +
-                               (inline_compact ? rule_sum(c2, n2, froot, up, g) : rule_scan(a, rec, kb, X, froot, up, g));
+            uint64_t const d =
+                uint64_t(n0) + (n1 - 1) + (c2ok ? rule_sum(c2, n2, froot, up, g) : lists.rule(froot, up, g));
             if (d) x1 = dec(f, d);
         }
-        if (inline_compact) {
-            uint64_t const keepm = __ballot(c == 0);
-            if (c == 0) ast(&rec[wpos + __popcll(keepm & lanemask_lt())], r);
-            wpos += __popcll(keepm);
-        }
+        // compaction in place: the writes stay below the records read so far
+        uint64_t const keepm = __ballot(c == 0);
+        if (c == 0) ast(&rec[wpos + __popcll(keepm & lanemask_lt())], r);
+        wpos += __popcll(keepm);
         xb.put(x1, f);
         xb.put(x2, e2);
-    }
-    if (!inline_compact) {
-        for (uint32_t j0 = 0; j0 < kb; j0 += kWave) {
-            uint32_t const j = j0 + lane;
-            int c = -1;
-            uint64_t r = 0;
-            if (j < kb) {
-                r = ald(&rec[j]);
-                c = classify(ald(&a.stamp[uint32_t(r >> 32)]), ald(&a.stamp[uint32_t(r)]), X);
-            }
-            uint64_t const keepm = __ballot(c == 0);
-            if (c == 0) ast(&rec[wpos + __popcll(keepm & lanemask_lt())], r);
-            wpos += __popcll(keepm);
-        }
     }
     if (lane == 0) {
         ast(&a.hdr[s], wpos | (up << 31));
@@ -669,8 +811,10 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
     __shared__ uint64_t s_c2[kWPB][kC2Cap];
     __shared__ uint32_t s_xl[kWPB][kXCap];
     __shared__ uint32_t s_xn[kWPB];
-    __shared__ uint32_t s_ws[kWPB][2][kWave];
-    __shared__ uint32_t s_incl[kBL], s_off[kBL];
+    __shared__ uint32_t s_wsi[kWPB][kWave];
+    __shared__ uint64_t s_wso[kWPB][kWave];
+    __shared__ uint32_t s_incl[kBL];
+    __shared__ uint64_t s_off[kBL];
     __shared__ uint32_t s_pfx[kMB + 1];
     __shared__ unsigned s_on;
     __shared__ int s_ok;
@@ -680,8 +824,9 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
     uint32_t const seg0 = uint32_t(uint64_t(a.m) * B / G), seg1 = uint32_t(uint64_t(a.m) * (B + 1) / G);
     MspCtl* ctl = a.ctl;
     if (tid == 0) s_on = 0;
-    WaveScan const ws{s_ws[wv][0], s_ws[wv][1]};
-    unsigned epoch = 0;
+    WaveScan const ws{s_wsi[wv], s_wso[wv]};
+    unsigned epoch = a.resume ? unsigned(ctl->r_epoch) : 0u;
+    if (a.slice_ticks && B == 0 && tid == 0) ctl->t_launch = wall_clock64();
     // claim list of sub-round Y filled by this block
     auto out_for = [&](uint32_t Y) {
         return Out{a.area + (size_t(Y % 3) * G + B) * kBlkCap, a.ovf + size_t(Y % 3) * a.nbloom, &ctl->ovf_n[Y % 3],
@@ -733,7 +878,7 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
     // ---- init: support-0 edges are final (process_support_zero); the rest
     // form this block's live segment; the first level is their min support
     // (slots of parity 1, so that level 0 can write parity 0 at once)
-    {
+    if (!a.resume) {
         uint32_t w = seg0;
         uint64_t mn = kInf;
         for (uint32_t base = seg0; base < seg1; base += kMB) {
@@ -757,9 +902,9 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
         }
         if (tid == 0) a.live_cnt[B] = w - seg0;
         set_slots(1, block_min_all(mn, s_red), kInf);
+        if (!grid_sync(a, G, ++epoch, &s_ok)) return;
     }
-    if (!grid_sync(a, G, ++epoch, &s_ok)) return;
-    uint32_t X = 2, k = 0;
+    uint32_t X = a.resume ? uint32_t(ctl->r_X) : 2u, k = a.resume ? uint32_t(ctl->r_k) : 0u;
     uint32_t nt = 0;            // claimed blooms of sub-round X
     uint64_t t_col = 0, t_pe = 0, w_col = 0, w_pe = 0, pmax_sum = 0;
 
@@ -802,10 +947,11 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
             dec.L = level_of(bg);
             // a bloom frees at most two edges per middle
             xb.ensure(2 * (kb < uint32_t(16 * kWave) ? kb : uint32_t(16 * kWave)), X + 1, a, out);
-            if (kb <= uint32_t(kWave)) process_bloom<1>(a, s, kb, up, r0, X, dec, c2, xb, out);
-            else if (kb <= uint32_t(4 * kWave)) process_bloom<4>(a, s, kb, up, r0, X, dec, c2, xb, out);
-            else if (kb <= uint32_t(16 * kWave)) process_bloom<16>(a, s, kb, up, r0, X, dec, c2, xb, out);
-            else process_large(a, s, kb, up, X, dec, c2, xb, out);
+            bool done = false;
+            if (kb <= uint32_t(kWave)) done = process_bloom<1>(a, s, kb, up, r0, X, dec, c2, xb, out);
+            else if (kb <= uint32_t(4 * kWave)) done = process_bloom<4>(a, s, kb, up, r0, X, dec, c2, xb, out);
+            else if (kb <= uint32_t(16 * kWave)) done = process_bloom<16>(a, s, kb, up, r0, X, dec, c2, xb, out);
+            if (!done) process_large(a, s, kb, up, X, dec, c2, xb, out);
         }
         tmin = dec.tmin;
         uint64_t const tw = wall_clock64() - tw0;
@@ -825,6 +971,7 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
             if (tid == 0) afetch_max(&ctl->pmax[X & 1], (unsigned long long)mx);
         }
         uint64_t const tp1 = wall_clock64();
+        if (a.slice_ticks && B == 0 && tid == 0) ast(&ctl->yield, tp1 - ctl->t_launch >= a.slice_ticks ? 1u : 0u);
         if (!grid_sync(a, G, ++epoch, &s_ok)) return ~0u;
         uint32_t const nn = open_list(X + 1);
         uint64_t const tp2 = wall_clock64();
@@ -843,8 +990,25 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
     };
 
     if constexpr (!kBatch) {
-        uint64_t L = level_min(1);
+        uint64_t L = a.resume ? ctl->r_L : level_min(1);
         while (L != kInf) {
+            // the slice is over: `yield` was set before the last barrier and
+            // is rewritten only after the next one, so every block agrees
+            if (a.slice_ticks && ald(&ctl->yield)) {
+                if (B == 0 && tid == 0) {
+                    ctl->r_X = X;
+                    ctl->r_k = k;
+                    ctl->r_L = L;
+                    ctl->r_epoch = epoch;
+                    ctl->t_collect += t_col;
+                    ctl->t_peel += t_pe;
+                    ctl->t_work_collect += w_col;
+                    ctl->t_work_peel += w_pe;
+                    ctl->pmax_sum += pmax_sum;
+                    ast(&ctl->yielded, 1u);
+                }
+                return;
+            }
             uint32_t const p = k & 1;
             // ---- collect: frontier of level L from this block's live segment
             {
@@ -1056,11 +1220,11 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
     if (B == 0 && tid == 0) {
         ctl->subrounds = X - 1;
         ctl->levels = k;
-        ctl->t_collect = t_col;
-        ctl->t_peel = t_pe;
-        ctl->t_work_collect = w_col;
-        ctl->t_work_peel = w_pe;
-        ctl->pmax_sum = pmax_sum;
+        ctl->t_collect += t_col;
+        ctl->t_peel += t_pe;
+        ctl->t_work_collect += w_col;
+        ctl->t_work_peel += w_pe;
+        ctl->pmax_sum += pmax_sum;
     }
 }
 
@@ -1107,6 +1271,16 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     BT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kMB, 0));
     if (per_cu < 1) return BT_ERR_DEVICE;
     uint32_t const nbloom = uint32_t(std::max<uint64_t>(ix.NB, 1));
+    // largest bloom: the waves' scratch for sorted both-frontier lists
+    uint32_t kmax = 0;
+    BT_HIP_TRY(hipMemcpyAsync(&kmax, dg->ctl + kCtlBloomMax, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    BT_HIP_TRY(hipStreamSynchronize(s));
+    // BT_PEEL_C2CAP (0..128) lowers the LDS list's share, to exercise the
+    // sorted lists on small graphs
+    uint32_t c2cap = uint32_t(kC2Cap);
+    if (const char* e = getenv("BT_PEEL_C2CAP")) c2cap = uint32_t(std::min<long>(std::max(0L, atol(e)), kC2Cap));
+    uint32_t const c2s_stride = kmax > c2cap ? (3 * kmax + 63) & ~63u : 0u;
+    if (c2s_stride) BT_TRY(bt_buf_reserve(ix.c2s, sizeof(uint32_t) * size_t(G) * kWPB * c2s_stride));
     // claim stamps: R real ones, then the dummies (both sized by the index build)
     if (ix.bclaim.bytes < sizeof(uint32_t) * (ix.R + size_t(G) * kClaimDummiesPerBlock)) return BT_ERR_INTERNAL;
     BT_TRY(bt_buf_reserve(ix.live, sizeof(uint32_t) * std::max<size_t>(m, 1)));
@@ -1131,7 +1305,7 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     a.sup = dg->sup;
     a.res = dg->res;
     a.stamp = dg->list0;
-    a.inc_off = static_cast<const uint32_t*>(ix.inc_off.ptr);
+    a.inc_off = static_cast<const uint64_t*>(ix.inc_off.ptr);
     a.inc = static_cast<const uint32_t*>(ix.inc.ptr);
     a.rec = static_cast<uint64_t*>(ix.vals1.ptr);
     a.hdr = static_cast<uint32_t*>(ix.bk.ptr);
@@ -1144,6 +1318,11 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     a.slots = reinterpret_cast<uint64_t*>(seg + o_slots);
     a.arrive = reinterpret_cast<uint32_t*>(seg + o_arrive);
     a.ctl = ctl;
+    a.c2s = c2s_stride ? static_cast<uint32_t*>(ix.c2s.ptr) : nullptr;
+    a.c2s_stride = c2s_stride;
+    a.c2cap = c2cap;
+    a.id_bits = 1;
+    while (a.id_bits < 32 && ((m ? m - 1 : 0) >> a.id_bits)) a.id_bits++;
     if (batch) {
         a.ngraph = dg->ngraph;
         a.egraph = dg->egraph;
@@ -1163,20 +1342,46 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     k_msp_init<<<grid_for(std::max<size_t>(n_arrive, n_glev), 256, 1u << 16), 256, 0, s>>>(
         ctl, a.arrive, uint32_t(n_arrive), a.glev, n_glev);
     BT_HIP_TRY(hipGetLastError());
-    // events 6..7 bracket the persistent kernel alone
+    // plain graphs run in time slices: the kernel returns at a level
+    // boundary once its slice is over and is launched again to resume
+    a.slice_ticks = (!batch && c->peel_slice_ms > 0) ? uint64_t(c->peel_slice_ms) * 100000ull : 0ull;
+    bool const progress = getenv("BT_PEEL_PROGRESS") != nullptr;
+    auto const t_first = std::chrono::steady_clock::now();
+    unsigned slices = 1;
+    // events 6..7 bracket the persistent kernel's launches
     BT_TRY(tm.start(6));
     void* params[] = {&a};
     {
         bt_cu_lease lease(c->device, c->cu_count, int(G));
-        if (G < unsigned(c->cu_count)) {
-            // Part of the device: the runtime runs cooperative launches one
-            // at a time, so this one is a plain launch. Its blocks are all
-            // resident together: each needs a CU of its own (LDS), the lease
-            // keeps the persistent peels' blocks within the CU count, and
-            // other kernels on the device finish by themselves.
-            BT_HIP_TRY(hipLaunchKernel(kern, dim3(G), dim3(kMB), params, 0, s));
-        } else {
-            BT_HIP_TRY(hipLaunchCooperativeKernel(kern, dim3(G), dim3(kMB), params, 0, s));
+        for (;; slices++) {
+            if (G < unsigned(c->cu_count)) {
+                // Part of the device: the runtime runs cooperative launches
+                // one at a time, so this one is a plain launch. Its blocks
+                // are all resident together: each needs a CU of its own
+                // (LDS), the lease keeps the persistent peels' blocks within
+                // the CU count, and other kernels on the device finish by
+                // themselves.
+                BT_HIP_TRY(hipLaunchKernel(kern, dim3(G), dim3(kMB), params, 0, s));
+            } else {
+                BT_HIP_TRY(hipLaunchCooperativeKernel(kern, dim3(G), dim3(kMB), params, 0, s));
+            }
+            if (!a.slice_ticks) break;
+            uint64_t st[2 + 4];  // yield, yielded, r_X, r_k, r_L, r_epoch
+            unsigned err = 0;
+            BT_HIP_TRY(hipMemcpyAsync(st, &ctl->yield, sizeof(st), hipMemcpyDeviceToHost, s));
+            BT_HIP_TRY(hipMemcpyAsync(&err, &ctl->err, sizeof(err), hipMemcpyDeviceToHost, s));
+            BT_HIP_TRY(hipStreamSynchronize(s));
+            if (err || !st[1]) break;
+            if (progress) {
+                double const sec =
+                    std::chrono::duration<double>(std::chrono::steady_clock::now() - t_first).count();
+                std::fprintf(stderr, "[bt_peel] slice %u: level %llu, level steps %llu, sub-rounds %llu, %.1f s\n",
+                             slices, (unsigned long long)st[4], (unsigned long long)st[3],
+                             (unsigned long long)st[2], sec);
+                std::fflush(stderr);
+            }
+            a.resume = 1;
+            BT_HIP_TRY(hipMemsetAsync(&ctl->yield, 0, 2 * sizeof(uint64_t), s));
         }
         BT_TRY(tm.start(7));
         BT_HIP_TRY(hipStreamSynchronize(s));
@@ -1218,7 +1423,8 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     c->last = count_t;
     c->last.peel_ms = tm.ms(5, 7);
     c->last.peel_kernel_ms = tm.ms(6, 7);
-    c->last.peel_kernel_launches = 1;
+    c->last.peel_kernel_launches = slices;
+    c->last.sorted_blooms = h.sorted;
     c->last.total_ms = count_t.total_ms + c->last.peel_ms;
     c->last.sub_rounds = h.subrounds;
     c->last.levels_scanned = h.levels;

@@ -219,7 +219,7 @@ int bt_ctx_create(int device, bt_ctx** out) {
     for (bt_buf* b : {&c->sort_tmp, &c->pool, &c->pinned_dummy, &c->ix.keys0, &c->ix.keys1, &c->ix.vals0,
                       &c->ix.vals1, &c->ix.bid, &c->ix.bstart, &c->ix.bk, &c->ix.bclaim, &c->ix.inc_off,
                       &c->ix.inc_cur, &c->ix.inc, &c->ix.touched, &c->ix.area, &c->ix.bgraph, &c->ix.glev,
-                      &c->ix.live, &c->ix.seg})
+                      &c->ix.live, &c->ix.seg, &c->ix.c2s})
         b->owner = c;
     if (hipHostMalloc(reinterpret_cast<void**>(&c->h_pinned), 64 * sizeof(uint64_t)) != hipSuccess) {
         bt_ctx_destroy(c);
@@ -238,7 +238,7 @@ int bt_ctx_destroy(bt_ctx* c) {
     bt_buf_free(c->pinned_dummy);
     bt_buf* ixb[] = {&c->ix.keys0, &c->ix.keys1, &c->ix.vals0, &c->ix.vals1, &c->ix.bid, &c->ix.bstart,
                      &c->ix.bk, &c->ix.bclaim, &c->ix.inc_off, &c->ix.inc_cur, &c->ix.inc, &c->ix.touched, &c->ix.area, &c->ix.bgraph, &c->ix.glev,
-                     &c->ix.live, &c->ix.seg};
+                     &c->ix.live, &c->ix.seg, &c->ix.c2s};
     for (bt_buf* b : ixb) bt_buf_free(*b);
     for (auto const& kv : c->cache) (void)hipFree(kv.second);
     c->cache.clear();
@@ -268,6 +268,14 @@ int bt_ctx_set_option(bt_ctx* ctx, int option, int64_t value) {
         case BT_OPT_PEEL_BLOCKS:
             if (value < 0 || value > ctx->cu_count) return BT_ERR_INVALID;
             ctx->peel_blocks = int(value);
+            return BT_OK;
+        case BT_OPT_BATCH_LANES:
+            if (value < 1 || value > 64) return BT_ERR_INVALID;
+            ctx->batch_lanes = int(value);
+            return BT_OK;
+        case BT_OPT_PEEL_SLICE_MS:
+            if (value < 0 || value > 3600000) return BT_ERR_INVALID;
+            ctx->peel_slice_ms = int(value);
             return BT_OK;
         case BT_OPT_BATCH_WEDGES:
             if (value < 0) return BT_ERR_INVALID;

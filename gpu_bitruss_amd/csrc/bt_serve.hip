@@ -209,6 +209,9 @@ int bt_serve_start(const bt_serve_config* cfg, bt_server** out) {
         st = bt_ctx_set_option(s->ctx_light, BT_OPT_PEEL_BLOCKS, light);
         if (st == BT_OK) st = bt_ctx_set_option(s->ctx_heavy, BT_OPT_PEEL_BLOCKS, cus - light);
         if (st == BT_OK) st = bt_ctx_set_option(s->ctx_large, BT_OPT_PEEL_BLOCKS, cus - light);
+        // a served batch is one union on its lane's CUs
+        if (st == BT_OK) st = bt_ctx_set_option(s->ctx_light, BT_OPT_BATCH_LANES, 1);
+        if (st == BT_OK) st = bt_ctx_set_option(s->ctx_heavy, BT_OPT_BATCH_LANES, 1);
     }
     auto destroy_ctx = [s] {
         if (s->ctx_light) bt_ctx_destroy(s->ctx_light);

@@ -68,9 +68,17 @@ class Device:
         _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(
             self.handle, _lib.OPT_BATCH_LEVELS, _lib.BATCH_LEVELS[mode]))
 
+    def set_batch_lanes(self, lanes: int) -> None:
+        """Lanes (contexts on equal CU shares) that batches uploaded afterwards run on."""
+        _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(self.handle, _lib.OPT_BATCH_LANES, lanes))
+
     def set_peel_blocks(self, cus: int) -> None:
         """CUs the persistent msp peel occupies (0: all)."""
         _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(self.handle, _lib.OPT_PEEL_BLOCKS, cus))
+
+    def set_peel_slice_ms(self, ms: int) -> None:
+        """Time slice of one persistent msp peel launch (0: one launch)."""
+        _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(self.handle, _lib.OPT_PEEL_SLICE_MS, ms))
 
     def set_index_mem_pct(self, pct: int) -> None:
         _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(self.handle, _lib.OPT_INDEX_MEM_PCT, pct))

@@ -39,13 +39,16 @@ typedef struct bt_timings {
     double count_ms;               /* butterfly / support counting           */
     double peel_ms;                /* msp peeling or hidx rounds             */
     double peel_kernel_ms;         /* sum of the peel kernel durations       */
-    uint64_t peel_kernel_launches; /* launches of the peel kernel            */
+    uint64_t peel_kernel_launches; /* launches of the peel kernel (msp: its
+                                      time slices)                           */
     double count_kernel_ms;        /* sum of the counting kernel durations   */
     uint64_t count_kernel_launches;
     uint64_t wedges_count;         /* priority wedges counted                */
     uint64_t wedges_peel;          /* reserved                               */
     uint64_t sub_rounds;           /* msp sub-rounds / hidx rounds           */
     uint64_t levels_scanned;       /* msp levels visited                     */
+    uint64_t sorted_blooms;        /* msp: bloom visits that sorted their
+                                      both-frontier middles                  */
 } bt_timings;
 
 /* Number of HIP devices (0 when none). Replaces cudaGetDeviceCount use
@@ -81,8 +84,9 @@ int bt_ctx_last_timings(const bt_ctx* ctx, bt_timings* out);
  *                          launch), else over the merge enumeration of the
  *                          reference's wedges; _MERGE / _INDEX force one path
  *                          (_INDEX fails with BT_ERR_BUDGET when it cannot run)
- *   BT_OPT_INDEX_MEM_PCT   share of free device memory the index may use
- *                          (1..95, default 50)
+ *   BT_OPT_INDEX_MEM_PCT   share of free device memory the emitted index
+ *                          records may use (1..95, default 50; the sorted
+ *                          index must then fit the free memory)
  *   BT_OPT_BATCH_WEDGES    batches uploaded afterwards put consecutive graphs
  *                          in one union up to this bound of priority wedges
  *                          (0, the default: as many as one bloom index holds)
@@ -96,9 +100,18 @@ int bt_ctx_last_timings(const bt_ctx* ctx, bt_timings* out);
  *   BT_OPT_PEEL_BLOCKS     CUs (one workgroup each) the persistent msp peel
  *                          occupies (0, the default: all). Peels of several
  *                          contexts run side by side while their CU counts
- *                          fit the device; otherwise they wait for CUs. */
+ *                          fit the device; otherwise they wait for CUs.
+ *   BT_OPT_PEEL_SLICE_MS   the persistent msp peel of a graph returns at the
+ *                          first level boundary after this many milliseconds
+ *                          and is launched again to resume (default 2000;
+ *                          0: one launch). Results are the same.
+ *   BT_OPT_BATCH_LANES     batches uploaded afterwards spread their graphs
+ *                          over this many lanes (1..64, default 2), each a
+ *                          context with its own stream and an equal share of
+ *                          the CUs for its persistent peels; the lanes run
+ *                          side by side. Results are the same. */
 enum { BT_OPT_MSP_PATH = 1, BT_OPT_INDEX_MEM_PCT = 2, BT_OPT_BATCH_WEDGES = 3, BT_OPT_BATCH_LEVELS = 4,
-       BT_OPT_PEEL_BLOCKS = 5 };
+       BT_OPT_PEEL_BLOCKS = 5, BT_OPT_PEEL_SLICE_MS = 6, BT_OPT_BATCH_LANES = 7 };
 enum { BT_MSP_PATH_AUTO = 0, BT_MSP_PATH_MERGE = 1, BT_MSP_PATH_INDEX = 2 };
 enum { BT_BATCH_LEVELS_PER_GRAPH = 0, BT_BATCH_LEVELS_UNION = 1 };
 int bt_ctx_set_option(bt_ctx* ctx, int option, int64_t value);
@@ -156,13 +169,16 @@ typedef struct bt_batch_result {
 } bt_batch_result;
 
 /* Copy k host graphs (an array of k structs) to the device as one batch.
- * Consecutive graphs share one union while a bound of their bloom records
- * (sum of C(deg, 2) over their vertices) fits a 32-bit index and the index
- * memory budget (BT_OPT_INDEX_MEM_PCT); a larger batch is held as several
- * unions, run one after the other. */
+ * The graphs are spread over BT_OPT_BATCH_LANES lanes by their priority
+ * wedges (a bound of their bloom records), largest first onto the lightest
+ * lane; a lane's graphs share one union while their wedges fit a 32-bit
+ * index and the lane's share of 85% of the free device memory (the lanes'
+ * indexes are built side by side, at most 52 bytes per wedge), else the
+ * lane holds several unions, run one after the other. The lanes run side
+ * by side. */
 int bt_batch_upload(bt_ctx* ctx, const bt_host_graph* graphs, size_t k, bt_batch** out);
 int bt_batch_free(bt_batch* b);
-/* Number of unions the batch was split into. */
+/* Number of unions the batch was split into (over all lanes). */
 int bt_batch_chunks(const bt_batch* b, size_t* chunks);
 
 /* Run `algo` (BT_ALGO_*) on every graph of the batch. `outs` (k pointers, or

@@ -15,7 +15,12 @@ the oracle CLI made separately with --hidx-from:
     python tests/golden/make_golden.py powerlaw_2m --skip-hidx \
         --hidx-from c2_hidx.json c2_hidx.u64
 
-(the .bin is the generated graph written with Graph.graph_to_bin).
+(the .bin is the generated graph written with Graph.graph_to_bin). The
+60M-edge skewed graph is recorded the same way, msp from a separate CLI run
+(--skip-msp leaves msp out, --msp-from merges the run):
+
+    python tests/golden/make_golden.py skewed_60m --skip-msp --skip-hidx \
+        --msp-from c3_msp.json c3_msp.u64
 """
 from __future__ import annotations
 
@@ -45,22 +50,34 @@ def main() -> None:
     ap.add_argument("workloads", nargs="+")
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 4)
     ap.add_argument("--skip-hidx", action="store_true")
+    ap.add_argument("--skip-msp", action="store_true")
+    ap.add_argument("--msp-from", nargs=2, metavar=("JSON", "U64"),
+                    help="oracle CLI output (JSON line, --dump array) of msp on the one workload")
     ap.add_argument("--hidx-from", nargs=2, metavar=("JSON", "U64"),
                     help="oracle CLI output (JSON line, --dump array) of hidx on the one workload")
     args = ap.parse_args()
-    if args.hidx_from and len(args.workloads) != 1:
-        ap.error("--hidx-from takes one workload")
+    if (args.hidx_from or args.msp_from) and len(args.workloads) != 1:
+        ap.error("--hidx-from and --msp-from take one workload")
     for wl in args.workloads:
         g = gb.Graph.generate(wl)
         sup, total, smax, st = oracle.ebfc(g, args.threads)
-        res, levels, st2 = oracle.msp(g, args.threads)
         out = {
             "workload": wl, "m": g.m, "n": g.n, "u_num": g.u_num, "l_num": g.l_num,
             "neighbors_sha256": hashlib.sha256(g.neighbors.tobytes()).hexdigest(),
             "butterflies": total, "support_max": smax, "ebfc_sha256": digest(sup),
-            "msp_levels": levels, "msp_sha256": digest(res),
-            "w_cnt": st["w_cnt"], "msp_stats": st2,
+            "w_cnt": st["w_cnt"],
         }
+        if not args.skip_msp:
+            res, levels, st2 = oracle.msp(g, args.threads)
+            out.update({"msp_levels": levels, "msp_sha256": digest(res), "msp_stats": st2})
+        if args.msp_from:
+            with open(args.msp_from[0]) as f:
+                run = json.loads(f.read().strip().splitlines()[-1])
+            r = np.fromfile(args.msp_from[1], dtype="<u8")
+            assert run["algo"] == "msp" and run["m"] == g.m and r.size == g.m, "msp run of another graph"
+            out.update({"msp_levels": run["iters"], "msp_sha256": digest(r),
+                        "msp_stats": {k: run[k] for k in ("w_peel", "i_peel", "b_peel", "d_peel",
+                                                          "sum_curr", "sub_rounds")}})
         if not args.skip_hidx:
             h, rounds, st3 = oracle.hidx(g, args.threads)
             out.update({"hidx_sha256": digest(h), "hidx_kmax": int(h.max()) if g.m else 0,

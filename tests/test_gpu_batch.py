@@ -77,7 +77,11 @@ def test_batch_in_many_unions(dev, algo):
     """A small per-union wedge bound splits the batch into many unions (a
     graph above the bound gets one of its own); results do not change."""
     gs = batch_graphs()
-    whole = dev.upload_batch(gs)
+    dev.set_batch_lanes(1)
+    try:
+        whole = dev.upload_batch(gs)
+    finally:
+        dev.set_batch_lanes(2)
     assert whole.chunks() == 1
     a1, s1 = whole.run(algo)
     whole.free()
@@ -92,6 +96,31 @@ def test_batch_in_many_unions(dev, algo):
     assert s1 == s2
     for x, y in zip(a1, a2):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("algo", ["hidx", "msp"])
+def test_batch_lanes_agree(dev, algo):
+    """One lane (one union on all CUs), three and eight lanes running side
+    by side on CU shares: the same per-graph results."""
+    gs = batch_graphs()
+    got = {}
+    try:
+        for lanes in (1, 3, 8):
+            dev.set_batch_lanes(lanes)
+            b = dev.upload_batch(gs)
+            say(f"{algo}: {lanes} lanes, {b.chunks()} unions")
+            assert b.chunks() >= lanes
+            got[lanes] = b.run(algo)
+            b.free()
+    finally:
+        dev.set_batch_lanes(2)
+    a1, s1 = got[1]
+    for lanes in (3, 8):
+        a2, s2 = got[lanes]
+        assert s1 == s2, lanes
+        for x, y in zip(a1, a2):
+            assert np.array_equal(x, y), lanes
 
 
 @pytest.mark.gpu

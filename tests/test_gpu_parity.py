@@ -196,6 +196,57 @@ def test_powerlaw_2m_golden_hidx(dev):
 
 
 @pytest.mark.gpu
+def test_msp_time_slices(dev):
+    """The persistent peel cut into 1 ms slices (each launch resumes at the
+    level boundary where the previous one returned) gives the golden
+    results of the 2M-edge graph."""
+    gold = json.load(open(os.path.join(GOLDEN, "powerlaw_2m.json")))
+    g = gb.Graph.generate("powerlaw_2m")
+    d = gb.Device(dev.device)
+    try:
+        d.set_peel_slice_ms(1)
+        dg = d.upload(g)
+        say("powerlaw_2m msp in 1 ms slices")
+        res, levels = dg.msp()
+        t = d.last_timings()
+        dg.free()
+    finally:
+        d.close()
+    say(f"slices: {t['peel_kernel_launches']}")
+    assert t["peel_kernel_launches"] > 1
+    assert digest(res) == gold["msp_sha256"] and levels == gold["msp_levels"]
+
+
+@pytest.mark.gpu
+def test_sorted_rule_lists(dev, monkeypatch):
+    """With the LDS list of both-frontier middles cut to nothing, every bloom
+    that needs the attribution rule sorts its both-frontier middles into id
+    lists and sums the rule by binary searches: results stay bit-exact
+    against the oracle and the 2M-edge golden."""
+    monkeypatch.setenv("BT_PEEL_C2CAP", "0")
+    sorted_visits = 0
+    graphs = [(f"quirk{s}", G.quirk_graph(s)) for s in G.QUIRK_SEEDS[:4] + G.ORDER_SEEDS[:2]]
+    graphs += [("mix2", gb.Graph.generate("mix_graph", 2)), ("hub", G.hub_graph(5, 6000, 20000, 400, 7000))]
+    for name, g in graphs:
+        say(f"sorted lists: {name} m={g.m}")
+        dg = dev.upload(g)
+        res, levels = dg.msp()
+        sorted_visits += dev.last_timings()["sorted_blooms"]
+        dg.free()
+        ores, olevels, _ = oracle.msp(g, 8)
+        assert np.array_equal(res, ores) and levels == olevels, name
+    gold = json.load(open(os.path.join(GOLDEN, "powerlaw_2m.json")))
+    dg = dev.upload(gb.Graph.generate("powerlaw_2m"))
+    say("sorted lists: powerlaw_2m")
+    res, levels = dg.msp()
+    sorted_visits += dev.last_timings()["sorted_blooms"]
+    dg.free()
+    say(f"bloom visits with sorted lists: {sorted_visits}")
+    assert digest(res) == gold["msp_sha256"] and levels == gold["msp_levels"]
+    assert sorted_visits > 0
+
+
+@pytest.mark.gpu
 def test_peel_on_part_of_the_device(dev):
     """The persistent peel on a subset of the CUs gives the same results,
     also with two contexts peeling side by side on disjoint CU counts."""
@@ -233,3 +284,23 @@ def test_peel_on_part_of_the_device(dev):
         assert np.array_equal(r, wr) and lv == wl
     for (r, lv), (wr, wl) in zip(got["b"], want[::-1]):
         assert np.array_equal(r, wr) and lv == wl
+
+
+@pytest.mark.gpu
+def test_skewed_60m_golden(dev):
+    """Butterflies and per-edge supports of the 60M-edge skewed graph
+    (4.67e9 priority wedges, 3.3e9 bloom records) against the oracle's
+    digests."""
+    gold = json.load(open(os.path.join(GOLDEN, "skewed_60m.json")))
+    g = gb.Graph.generate("skewed_60m")
+    assert g.m == gold["m"]
+    assert hashlib.sha256(g.neighbors.tobytes()).hexdigest() == gold["neighbors_sha256"]
+    dg = dev.upload(g)
+    say("skewed_60m bfc")
+    assert dg.bfc() == gold["butterflies"]
+    say("skewed_60m ebfc")
+    sup, total, smax = dg.ebfc()
+    assert total == gold["butterflies"] and smax == gold["support_max"]
+    assert digest(sup) == gold["ebfc_sha256"]
+    assert dev.last_timings()["wedges_count"] == gold["w_cnt"]
+    dg.free()
