@@ -24,6 +24,10 @@
 // concatenation of the edge's blooms 64 middles at a time.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+
 #include <algorithm>
 
 #include "bt_common.h"
@@ -60,7 +64,23 @@ struct HidxArgs {
     const uint32_t* egraph;  // batches: graph of each edge (else null)
     uint32_t* glast;         // batches: per graph, last round with a change
     uint32_t round;
+    // per bloom: a value of one of its edges changed in the previous round
+    // (null: every bloom, the first round), and the marks of this round
+    const uint8_t* dirty_in;
+    uint8_t* dirty_out;
 };
+
+// Whether any bloom holding edge e (records q0..q1 of its incidence list)
+// is marked in dirty (whole wave).
+__device__ __forceinline__ bool any_dirty(const HidxArgs& a, const uint8_t* dirty, uint64_t q0, uint64_t q1) {
+    int const lane = threadIdx.x & (kWave - 1);
+    for (uint64_t b0 = q0; b0 < q1; b0 += kWave) {
+        uint64_t const q = b0 + lane;
+        bool const d = q < q1 && dirty[a.bid[a.inc[q]] - 1];
+        if (__ballot(d)) return true;
+    }
+    return false;
+}
 
 // Streams the values of edge e's butterflies (one wave), clips them to cap
 // (= prev[e], which bounds the answer) and bins those in [lo, hi] into
@@ -134,6 +154,12 @@ __global__ __launch_bounds__(kHT) void k_hidx_round(HidxArgs a) {
             continue;
         }
         uint64_t const q0 = a.inc_off[e], q1 = a.inc_off[e + 1];
+        // no bloom of e saw a change in the previous round: the values of
+        // e's butterflies, and so its h-index, are those of that round
+        if (a.dirty_in && !any_dirty(a, a.dirty_in, q0, q1)) {
+            if (lane == 0) a.curr[e] = pe;
+            continue;
+        }
         // candidates [lo, hi]; `above` values lie above hi
         uint64_t lo = 0, hi = pe, above = 0;
         uint64_t width = (pe + kHB) / kHB;  // ceil((pe + 1) / kHB)
@@ -189,6 +215,7 @@ __global__ __launch_bounds__(kHT) void k_hidx_round(HidxArgs a) {
         if (lane == 0) a.curr[e] = h;
         if (h != pe) {
             changed = true;
+            for (uint64_t q = q0 + lane; q < q1; q += kWave) a.dirty_out[a.bid[a.inc[q]] - 1] = 1;
             if (a.egraph && a.egraph[e] != last_g) {  // batches: per-graph round count
                 last_g = a.egraph[e];
                 if (lane == 0) __hip_atomic_store(&a.glast[last_g], a.round, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -223,9 +250,16 @@ int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds_out) {
         unsigned const grid_mv = grid_for(std::max<uint32_t>(R, 1), 256, unsigned(c->cu_count) * 16);
         unsigned const grid_e = std::max(1u, std::min(unsigned(c->cu_count) * 2, (m + kHW - 1) / kHW));
         uint64_t* h_changed = static_cast<uint64_t*>(c->h_pinned);
+        // per-bloom change marks of the previous and the current round
+        size_t const nbm = std::max<size_t>(ix.NB, 1);
+        BT_TRY(bt_buf_reserve(ix.hdirty, 2 * nbm));
+        uint8_t* dirty[2] = {static_cast<uint8_t*>(ix.hdirty.ptr), static_cast<uint8_t*>(ix.hdirty.ptr) + nbm};
+        bool const progress = getenv("BT_PEEL_PROGRESS") != nullptr;
+        auto const t_first = std::chrono::steady_clock::now();
         for (;;) {
             rounds++;
             BT_HIP_TRY(hipMemsetAsync(changed, 0, sizeof(uint64_t), s));
+            BT_HIP_TRY(hipMemsetAsync(dirty[rounds & 1], 0, nbm, s));
             if (R) k_hidx_mv<<<grid_mv, 256, 0, s>>>(R, static_cast<const uint64_t*>(ix.vals1.ptr), prev, mv);
             HidxArgs a{m,
                        prev,
@@ -239,12 +273,21 @@ int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds_out) {
                        changed,
                        dg->egraph,
                        dg->graph_last,
-                       rounds};
+                       rounds,
+                       rounds > 1 ? dirty[(rounds + 1) & 1] : nullptr,
+                       dirty[rounds & 1]};
             k_hidx_round<<<grid_e, kHT, 0, s>>>(a);
             BT_HIP_TRY(hipGetLastError());
             BT_HIP_TRY(hipMemcpyAsync(h_changed, changed, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
             BT_HIP_TRY(hipStreamSynchronize(s));
             std::swap(prev, curr);
+            if (progress) {
+                double const sec =
+                    std::chrono::duration<double>(std::chrono::steady_clock::now() - t_first).count();
+                std::fprintf(stderr, "[bt_hidx] round %u: %s, %.1f s\n", rounds,
+                             *h_changed ? "values changed" : "no change", sec);
+                std::fflush(stderr);
+            }
             if (!*h_changed) break;
         }
         if (prev != dg->res)

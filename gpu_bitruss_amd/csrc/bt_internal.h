@@ -76,6 +76,7 @@ struct bt_index {
     bt_buf live;          // u32 per edge: per-block live-edge segments
     bt_buf seg;           // u32 per block + 1: segment offsets, then counts
     bt_buf c2s;           // u32 per peel wave: sorted both-frontier lists
+    bt_buf hdirty;        // u8 2 x blooms: h-index rounds' change marks
     uint64_t R = 0, NB = 0;
 };
 

@@ -97,6 +97,7 @@ struct MspCtl {
     unsigned long long r_X, r_k, r_L, r_epoch;
     unsigned long long t_launch;     // block 0's clock at the launch
     unsigned long long sorted;       // bloom visits that sorted both-frontier lists
+    unsigned long long live_n[2];    // plain graphs: lengths of the two live-edge lists
 };
 static_assert(sizeof(MspCtl) <= 128 * sizeof(uint64_t), "control block overlays dg->ctl");
 
@@ -818,6 +819,8 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
     __shared__ uint32_t s_pfx[kMB + 1];
     __shared__ unsigned s_on;
     __shared__ int s_ok;
+    __shared__ uint32_t s_gbase;
+    __shared__ unsigned s_pick;
     uint32_t const G = gridDim.x, B = blockIdx.x, tid = threadIdx.x;
     int const wv = tid / kWave;
     uint32_t const gw = B * kWPB + wv, nw = G * kWPB;
@@ -876,8 +879,10 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
     };
 
     // ---- init: support-0 edges are final (process_support_zero); the rest
-    // form this block's live segment; the first level is their min support
-    // (slots of parity 1, so that level 0 can write parity 0 at once)
+    // are live: in this block's segment of the live array (batches), or
+    // appended to live list 0 (plain graphs: every level's collect then
+    // spreads the list evenly over the blocks); the first level is their
+    // min support (slots of parity 1, so that level 0 can write parity 0)
     if (!a.resume) {
         uint32_t w = seg0;
         uint64_t mn = kInf;
@@ -897,7 +902,14 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
             }
             uint32_t tot = 0;
             uint32_t const ex = block_excl_scan<kMB>(keep ? 1u : 0u, s_part, tot);
-            if (keep) a.live[w + ex] = i;
+            if constexpr (kBatch) {
+                if (keep) a.live[w + ex] = i;
+            } else {
+                if (tid == 0) s_gbase = tot ? uint32_t(afetch_add(&ctl->live_n[0], (unsigned long long)tot)) : 0u;
+                __syncthreads();
+                if (keep) ast(&a.live[s_gbase + ex], i);  // read by other blocks
+                __syncthreads();
+            }
             w += tot;
         }
         if (tid == 0) a.live_cnt[B] = w - seg0;
@@ -920,27 +932,42 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
         CrossBuf xb{s_xl[wv], 0, ws};
         int const lane = tid & (kWave - 1);
         uint64_t const tw0 = wall_clock64();
-        // software pipeline over this wave's blooms: the list entry two
-        // blooms ahead and the header and first 64 middles of the next one
-        // load while the current bloom is processed
+        // the block's blooms are list entries B, B + G, B + 2G, ...: each
+        // wave starts with one and takes the next slot from an LDS counter,
+        // so a wave held up by a large bloom leaves the rest to the others.
+        // Software pipeline: the list entry two blooms ahead and the header
+        // and first 64 middles of the next one load while the current bloom
+        // is processed.
+        uint32_t const nb_blk = nt > B ? (nt - B + G - 1) / G : 0u;
+        if (tid == 0) s_pick = kWPB;
+        __syncthreads();
+        auto grab = [&]() -> uint32_t {
+            uint32_t v = 0;
+            if (lane == 0) v = atomicAdd(&s_pick, 1u);
+            return __builtin_amdgcn_readfirstlane(v);
+        };
         uint32_t s_next = 0, s_next2 = 0, hdr_next = 0, bg_next = 0;
         uint64_t r0_next = 0;
-        if (gw < nt) {
-            s_next = list_entry(X, gw);
+        uint32_t j_nxt = grab();
+        bool more = uint32_t(wv) < nb_blk;
+        if (more) {
+            s_next = list_entry(X, B + G * uint32_t(wv));
             hdr_next = ald(&a.hdr[s_next]);
             r0_next = (s_next + lane < a.R) ? ald(&a.rec[s_next + lane]) : 0;
             if constexpr (kBatch) bg_next = a.bgraph[s_next];
         }
-        if (gw + nw < nt) s_next2 = list_entry(X, gw + nw);
-        for (uint32_t ti = gw; ti < nt; ti += nw) {
+        if (j_nxt < nb_blk) s_next2 = list_entry(X, B + G * j_nxt);
+        while (more) {
             uint32_t const s = s_next, hdr = hdr_next, bg = bg_next;
             uint64_t const r0 = r0_next;
-            if (ti + nw < nt) {
+            more = j_nxt < nb_blk;
+            if (more) {
                 s_next = s_next2;
                 hdr_next = ald(&a.hdr[s_next]);
                 r0_next = (s_next + lane < a.R) ? ald(&a.rec[s_next + lane]) : 0;
                 if constexpr (kBatch) bg_next = a.bgraph[s_next];
-                if (ti + 2 * nw < nt) s_next2 = list_entry(X, ti + 2 * nw);
+                j_nxt = grab();
+                if (j_nxt < nb_blk) s_next2 = list_entry(X, B + G * j_nxt);
             }
             uint32_t const kb = hdr & 0x7FFFFFFFu, up = hdr >> 31;
             if (kb < 2) continue;
@@ -1015,12 +1042,16 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
                 uint64_t const tp0 = wall_clock64();
                 if (B == 0 && tid == 0) ast(&ctl->ovf_n[(X + 1) % 3], 0u);
                 Out const out = out_for(X);
-                uint32_t const cnt = a.live_cnt[B];
-                uint32_t w = seg0;
+                // live list p is read in chunks spread over all blocks, the
+                // edges that stay live are appended to list 1 - p (device-
+                // scope accesses: other blocks, on other XCDs, read them)
+                const uint32_t* cur = a.live + size_t(p) * a.m;
+                uint32_t* nxt = a.live + size_t(p ^ 1u) * a.m;
+                uint32_t const cnt = uint32_t(ald(&ctl->live_n[p]));
                 uint64_t mn = kInf;
                 CrossBuf xb{s_xl[wv], 0, ws};
                 constexpr int U = 4;
-                for (uint32_t base = seg0; base < seg0 + cnt; base += kMB * U) {
+                for (uint32_t base = B * (kMB * U); base < cnt; base += G * (kMB * U)) {
                     xb.ensure(U * kWave, X, a, out);
                     // an edge finished at an earlier level has support below L
                     // (it was peeled at a level < L with support <= that level);
@@ -1030,29 +1061,31 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
 #pragma unroll
                     for (int u = 0; u < U; u++) {
                         uint32_t const i = base + u * kMB + tid;
-                        e[u] = i < seg0 + cnt ? a.live[i] : 0u;
+                        e[u] = i < cnt ? ald(&cur[i]) : 0u;
                     }
 #pragma unroll
                     for (int u = 0; u < U; u++) {
                         uint32_t const i = base + u * kMB + tid;
-                        v[u] = i < seg0 + cnt ? ald(&a.sup[e[u]]) : 0ull;
+                        v[u] = i < cnt ? ald(&a.sup[e[u]]) : 0ull;
                     }
                     uint32_t nk = 0;
 #pragma unroll
                     for (int u = 0; u < U; u++) {
                         uint32_t const i = base + u * kMB + tid;
-                        bool const in = i < seg0 + cnt;
+                        bool const in = i < cnt;
                         nk += (in && v[u] > L) ? 1u : 0u;
                         if (in && v[u] > L) mn = v[u] < mn ? v[u] : mn;
                     }
                     uint32_t tot = 0;
-                    uint32_t pos = w + block_excl_scan<kMB>(nk, s_part, tot);
-                    w += tot;
+                    uint32_t const ex = block_excl_scan<kMB>(nk, s_part, tot);
+                    if (tid == 0) s_gbase = tot ? uint32_t(afetch_add(&ctl->live_n[p ^ 1u], (unsigned long long)tot)) : 0u;
+                    __syncthreads();
+                    uint32_t pos = s_gbase + ex;
 #pragma unroll
                     for (int u = 0; u < U; u++) {
                         uint32_t const i = base + u * kMB + tid;
-                        bool const in = i < seg0 + cnt;
-                        if (in && v[u] > L) a.live[pos++] = e[u];
+                        bool const in = i < cnt;
+                        if (in && v[u] > L) ast(&nxt[pos++], e[u]);
                         bool const take = in && v[u] == L;
                         if (take) {
                             ast(&a.stamp[e[u]], X);
@@ -1060,8 +1093,8 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
                         }
                         xb.put(take, e[u]);
                     }
+                    __syncthreads();
                 }
-                if (tid == 0) a.live_cnt[B] = w - seg0;
                 set_slots(p, block_min_all(mn, s_red), kInf);
                 if ((tid & (kWave - 1)) == 0) s_xn[wv] = xb.n;
                 __syncthreads();
@@ -1069,6 +1102,9 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
                 close_out(X);
                 uint64_t const tp1 = wall_clock64();
                 if (!grid_sync(a, G, ++epoch, &s_ok)) return;
+                // every block has read list p's length: it is the next
+                // level's append target (the peel barriers come between)
+                if (B == 0 && tid == 0) ast(&ctl->live_n[p], 0ull);
                 nt = open_list(X);
                 uint64_t const tp2 = wall_clock64();
                 t_col += tp2 - tp0;
@@ -1283,7 +1319,7 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     if (c2s_stride) BT_TRY(bt_buf_reserve(ix.c2s, sizeof(uint32_t) * size_t(G) * kWPB * c2s_stride));
     // claim stamps: R real ones, then the dummies (both sized by the index build)
     if (ix.bclaim.bytes < sizeof(uint32_t) * (ix.R + size_t(G) * kClaimDummiesPerBlock)) return BT_ERR_INTERNAL;
-    BT_TRY(bt_buf_reserve(ix.live, sizeof(uint32_t) * std::max<size_t>(m, 1)));
+    BT_TRY(bt_buf_reserve(ix.live, sizeof(uint32_t) * 2 * std::max<size_t>(m, 1)));
     BT_TRY(bt_buf_reserve(ix.area, sizeof(uint32_t) * 3 * size_t(G) * kBlkCap));
     BT_TRY(bt_buf_reserve(ix.touched, sizeof(uint32_t) * 3 * size_t(nbloom)));
     // per-block words: live counts, list lengths, level slots, barrier slots

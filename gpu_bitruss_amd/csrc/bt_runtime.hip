@@ -219,7 +219,7 @@ int bt_ctx_create(int device, bt_ctx** out) {
     for (bt_buf* b : {&c->sort_tmp, &c->pool, &c->pinned_dummy, &c->ix.keys0, &c->ix.keys1, &c->ix.vals0,
                       &c->ix.vals1, &c->ix.bid, &c->ix.bstart, &c->ix.bk, &c->ix.bclaim, &c->ix.inc_off,
                       &c->ix.inc_cur, &c->ix.inc, &c->ix.touched, &c->ix.area, &c->ix.bgraph, &c->ix.glev,
-                      &c->ix.live, &c->ix.seg, &c->ix.c2s})
+                      &c->ix.live, &c->ix.seg, &c->ix.c2s, &c->ix.hdirty})
         b->owner = c;
     if (hipHostMalloc(reinterpret_cast<void**>(&c->h_pinned), 64 * sizeof(uint64_t)) != hipSuccess) {
         bt_ctx_destroy(c);
@@ -238,7 +238,7 @@ int bt_ctx_destroy(bt_ctx* c) {
     bt_buf_free(c->pinned_dummy);
     bt_buf* ixb[] = {&c->ix.keys0, &c->ix.keys1, &c->ix.vals0, &c->ix.vals1, &c->ix.bid, &c->ix.bstart,
                      &c->ix.bk, &c->ix.bclaim, &c->ix.inc_off, &c->ix.inc_cur, &c->ix.inc, &c->ix.touched, &c->ix.area, &c->ix.bgraph, &c->ix.glev,
-                     &c->ix.live, &c->ix.seg, &c->ix.c2s};
+                     &c->ix.live, &c->ix.seg, &c->ix.c2s, &c->ix.hdirty};
     for (bt_buf* b : ixb) bt_buf_free(*b);
     for (auto const& kv : c->cache) (void)hipFree(kv.second);
     c->cache.clear();
