@@ -52,14 +52,19 @@ __global__ void k_bloom_size(uint32_t nbloom, uint32_t R, int nb, const uint64_t
                              uint32_t* __restrict__ bgraph, uint32_t* __restrict__ kmax) {
     uint32_t const b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b == 0) bstart[nbloom] = R;
-    if (b >= nbloom) return;
-    uint32_t const s = bstart[b];
-    uint32_t const e = (b + 1 < nbloom) ? bstart[b + 1] : R;
-    uint32_t const r = uint32_t(keys[s] >> nb);
-    bk[s] = (e - s) | (uint32_t(r_up[r]) << 31);
-    atomicMax(kmax, e - s);
-    bclaim[s] = 0;
-    if (egraph) bgraph[s] = egraph[uint32_t(rec[s] >> 32)];  // batches: the bloom's graph
+    uint32_t size = 0;
+    if (b < nbloom) {
+        uint32_t const s = bstart[b];
+        uint32_t const e = (b + 1 < nbloom) ? bstart[b + 1] : R;
+        uint32_t const r = uint32_t(keys[s] >> nb);
+        size = e - s;
+        bk[s] = size | (uint32_t(r_up[r]) << 31);
+        bclaim[s] = 0;
+        if (egraph) bgraph[s] = egraph[uint32_t(rec[s] >> 32)];  // batches: the bloom's graph
+    }
+    // largest bloom: one atomic per wave
+    uint64_t const mx = wave_max_u64(size);
+    if ((threadIdx.x & (kWave - 1)) == 0 && mx) atomicMax(kmax, uint32_t(mx));
 }
 
 __global__ void k_inc_count(uint64_t R, const uint64_t* __restrict__ rec, uint32_t* __restrict__ cnt) {

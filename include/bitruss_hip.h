@@ -106,7 +106,7 @@ int bt_ctx_last_timings(const bt_ctx* ctx, bt_timings* out);
  *                          and is launched again to resume (default 2000;
  *                          0: one launch). Results are the same.
  *   BT_OPT_BATCH_LANES     batches uploaded afterwards spread their graphs
- *                          over this many lanes (1..64, default 2), each a
+ *                          over this many lanes (1..64, default 4), each a
  *                          context with its own stream and an equal share of
  *                          the CUs for its persistent peels; the lanes run
  *                          side by side. Results are the same. */
