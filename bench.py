@@ -131,7 +131,7 @@ def bench_mix(args) -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
         "config": {"workload": f"mix: {k} bipartite graphs, seeds 1000..{999 + k}, |U|,|V| log-uniform in "
                                "[50, 5000], density log-uniform in [0.5%, 5%], even seeds uniform, odd Chung-Lu 2.2",
-                   "m": m_total, "algo": "msp", "lanes": args.lanes or 4},
+                   "m": m_total, "algo": "msp", "lanes": args.lanes or 2},
         "plain_graphs_per_s": k / tp, "plain_s": tp, "speedup": tp / t, "identical": same,
     }), flush=True)
     return 0
@@ -213,7 +213,7 @@ def main() -> int:
                     help="powerlaw_2m (the headline line), mix (batched vs plain) or serve")
     ap.add_argument("--graphs", type=int, default=4096, help="mix: number of graphs")
     ap.add_argument("--check", action="store_true", help="mix: compare every graph with the plain path")
-    ap.add_argument("--lanes", type=int, default=0, help="mix: batch lanes (0: the library default, 4)")
+    ap.add_argument("--lanes", type=int, default=0, help="mix: batch lanes (0: the library default, 2)")
     ap.add_argument("--requests", type=int, default=400, help="serve: requests per offered rate")
     ap.add_argument("--qps", default="25,100,400", help="serve: offered request rates")
     args = ap.parse_args()

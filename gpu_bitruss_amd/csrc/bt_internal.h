@@ -98,7 +98,7 @@ struct bt_ctx {
     double index_mem_frac = 0.5;   // share of free device memory the index may use
     uint64_t batch_wedges = 0;     // wedge bound per batch union (0: the index limits)
     int batch_levels = 1;          // BT_BATCH_LEVELS_*: how a union's msp walks levels
-    int batch_lanes = 4;           // batches: lanes (contexts on CU shares) run side by side
+    int batch_lanes = 2;           // batches: lanes (contexts on CU shares) run side by side
     int peel_blocks = 0;           // CUs the persistent peel uses (0: all)
     int peel_slice_ms = 2000;      // time slice of a persistent peel launch (0: one launch)
     // device blocks given back by freed graphs and batches, reused by the

@@ -106,10 +106,13 @@ int bt_ctx_last_timings(const bt_ctx* ctx, bt_timings* out);
  *                          and is launched again to resume (default 2000;
  *                          0: one launch). Results are the same.
  *   BT_OPT_BATCH_LANES     batches uploaded afterwards spread their graphs
- *                          over this many lanes (1..64, default 4), each a
+ *                          over this many lanes (1..64, default 2), each a
  *                          context with its own stream and an equal share of
  *                          the CUs for its persistent peels; the lanes run
- *                          side by side. Results are the same. */
+ *                          side by side. Results are the same. Streams beyond
+ *                          the process's hardware queues (GPU_MAX_HW_QUEUES,
+ *                          4 by default) share queues, which runs their
+ *                          kernels one after the other. */
 enum { BT_OPT_MSP_PATH = 1, BT_OPT_INDEX_MEM_PCT = 2, BT_OPT_BATCH_WEDGES = 3, BT_OPT_BATCH_LEVELS = 4,
        BT_OPT_PEEL_BLOCKS = 5, BT_OPT_PEEL_SLICE_MS = 6, BT_OPT_BATCH_LANES = 7 };
 enum { BT_MSP_PATH_AUTO = 0, BT_MSP_PATH_MERGE = 1, BT_MSP_PATH_INDEX = 2 };

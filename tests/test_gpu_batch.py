@@ -81,7 +81,7 @@ def test_batch_in_many_unions(dev, algo):
     try:
         whole = dev.upload_batch(gs)
     finally:
-        dev.set_batch_lanes(4)
+        dev.set_batch_lanes(2)
     assert whole.chunks() == 1
     a1, s1 = whole.run(algo)
     whole.free()
@@ -114,7 +114,7 @@ def test_batch_lanes_agree(dev, algo):
             got[lanes] = b.run(algo)
             b.free()
     finally:
-        dev.set_batch_lanes(4)
+        dev.set_batch_lanes(2)
     a1, s1 = got[1]
     for lanes in (3, 8):
         a2, s2 = got[lanes]
