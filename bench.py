@@ -163,10 +163,11 @@ def bench_serve(args) -> int:
     t_plain = time.perf_counter() - t0
     log(f"plain path, one graph after the other (upload included): {t_plain:.1f} s")
     dev.close()
+    cus = tuple(int(x) for x in args.serve_cus.split(","))
     levels = []
     for qps in [float(q) for q in args.qps.split(",")]:
         gaps = rng.exponential(1.0 / qps, n)
-        with gb.Server(large_edges=1_000_000) as srv:
+        with gb.Server(large_edges=1_000_000, batch_wedges=args.serve_batch_wedges, cus=cus) as srv:
             tickets = [None] * n
             start = time.perf_counter()
 
@@ -195,7 +196,8 @@ def bench_serve(args) -> int:
         "unit": "graphs/s", "n_gpus": 1, "steps": len(levels), "warmup": 0, "ms_per_step": None,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
         "config": {"workload": f"serve: {n} requests, 98% mix graphs, 2% Chung-Lu 2.2 with m in [1M, 5M], "
-                               "budget 64 GiB", "algo": "msp"},
+                               "budget 64 GiB", "algo": "msp", "lane_cus": list(cus),
+                   "batch_wedges": args.serve_batch_wedges or 100_000_000},
         "plain_graphs_per_s": n / t_plain, "levels": levels,
     }), flush=True)
     return 0
@@ -216,6 +218,10 @@ def main() -> int:
     ap.add_argument("--lanes", type=int, default=0, help="mix: batch lanes (0: the library default, 2)")
     ap.add_argument("--requests", type=int, default=400, help="serve: requests per offered rate")
     ap.add_argument("--qps", default="25,100,400", help="serve: offered request rates")
+    ap.add_argument("--serve-cus", default="0,0,0",
+                    help="serve: CUs of the light, heavy and large lanes (0: the library defaults)")
+    ap.add_argument("--serve-batch-wedges", type=int, default=0,
+                    help="serve: priority wedges per heavy batch at most (0: the library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "

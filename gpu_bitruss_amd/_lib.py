@@ -82,6 +82,10 @@ class BtServeConfig(C.Structure):
         ("large_edges", C.c_uint32),
         ("max_batch", C.c_uint32),
         ("light_wedges", C.c_uint64),
+        ("batch_wedges", C.c_uint64),
+        ("light_cus", C.c_uint32),
+        ("heavy_cus", C.c_uint32),
+        ("large_cus", C.c_uint32),
     ]
 
 

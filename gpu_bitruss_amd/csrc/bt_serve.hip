@@ -7,18 +7,21 @@
 //     cfg.light_wedges); their worker takes every pending light request of
 //     the oldest request's algorithm that fits the memory budget (up to
 //     cfg.max_batch) and runs them as one batch (bt_batch_*); requests that
-//     arrive meanwhile form the next batch. Its persistent peel occupies a
-//     quarter of the CUs, so light requests are not held up by heavy ones;
-//   - heavy: the other small graphs, batched the same way on the other
-//     three quarters of the CUs;
-//   - large: graphs with at least cfg.large_edges edges, run one at a time
-//     on the plain path (bt_graph_upload + the algorithm), sharing the
-//     heavy lane's CUs.
+//     arrive meanwhile form the next batch;
+//   - heavy: the other small graphs, batched the same way, a batch holding
+//     at most cfg.batch_wedges priority wedges so that it stays short (a
+//     batch ends when its slowest graph does);
+//   - large: graphs with at least cfg.large_edges edges, batched the same
+//     way (a union of large graphs costs about as many phases as its
+//     largest graph, so concurrent large requests finish together instead
+//     of one after the other).
 // Each lane has its own context (HIP stream); the lanes' kernels overlap on
 // the device, and their persistent peels run side by side on disjoint CU
-// counts (bt_cu_lease). Admission: a request starts only when the estimated
-// bytes of all running requests plus its own stay within cfg.budget_bytes;
-// requests wait in arrival order per lane.
+// shares (bt_cu_lease; a quarter for the light lane, three eighths for the
+// heavy lane and the rest for large requests by default), so a large
+// request does not hold up the batches. Admission: a request starts only
+// when the estimated bytes of all running requests plus its own stay within
+// cfg.budget_bytes; requests wait in arrival order per lane.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -46,6 +49,7 @@ struct Request {
     bt_host_graph g{};                  // owned copy
     std::vector<uint32_t> arrays;       // storage of g's arrays
     uint64_t est_bytes = 0;
+    uint64_t wedges = 0;                // priority wedges
     uint64_t run_bytes = 0;             // the run's fixed share, on one request of it
     std::vector<uint64_t> out;          // per-edge result
     bt_batch_result res{};
@@ -84,9 +88,9 @@ struct bt_server {
 
     // takes the next group of requests from q that fits the budget (caller
     // holds mu); empty when the front does not fit yet
-    std::vector<RequestPtr> take(std::deque<RequestPtr>& q, bool batch) {
+    std::vector<RequestPtr> take(std::deque<RequestPtr>& q, bool batch, uint64_t max_wedges) {
         std::vector<RequestPtr> got;
-        uint64_t add = kRunBytes;
+        uint64_t add = kRunBytes, wedges = 0;
         int const algo = q.empty() ? 0 : q.front()->algo;
         for (auto it = q.begin(); it != q.end() && got.size() < (batch ? cfg.max_batch : 1u);) {
             RequestPtr const& r = *it;
@@ -95,6 +99,8 @@ struct bt_server {
                 continue;
             }
             if (in_use + add + r->est_bytes > cfg.budget_bytes) break;
+            if (!got.empty() && wedges + r->wedges > max_wedges) break;
+            wedges += r->wedges;
             add += r->est_bytes;
             got.push_back(r);
             it = q.erase(it);
@@ -126,20 +132,21 @@ struct bt_server {
     // waits for work in q (or the end); false when the server stops
     bool next(std::deque<RequestPtr>& q, bool batch, std::vector<RequestPtr>& rs) {
         std::unique_lock<std::mutex> lk(mu);
+        uint64_t const max_wedges = &q == &heavy_q ? cfg.batch_wedges : ~0ull;
         for (;;) {
             if (!q.empty()) {
-                rs = take(q, batch);
+                rs = take(q, batch, max_wedges);
                 if (!rs.empty()) break;
             } else if (stopping) {
                 return false;
             }
             cv_work.wait(lk);
         }
-        if (batch) {
+        if (&q == &large_q) {
+            stats.large_runs++;
+        } else {
             stats.batches++;
             stats.batched_requests += rs.size();
-        } else {
-            stats.large_runs++;
         }
         return true;
     }
@@ -165,26 +172,6 @@ struct bt_server {
         }
     }
 
-    void run_large() {
-        std::vector<RequestPtr> rs;
-        while (next(large_q, false, rs)) {
-            Request& r = *rs.front();
-            bt_dgraph* dg = nullptr;
-            int st = bt_graph_upload(ctx_large, &r.g, &dg);
-            if (st == BT_OK) {
-                r.out.assign(r.algo == BT_ALGO_BFC ? 0 : r.g.m, 0);
-                uint64_t* out = r.out.empty() ? nullptr : r.out.data();
-                switch (r.algo) {
-                    case BT_ALGO_BFC: st = bt_bfc(ctx_large, dg, &r.res.total); break;
-                    case BT_ALGO_EBFC: st = bt_ebfc(ctx_large, dg, out, &r.res.total, &r.res.support_max); break;
-                    case BT_ALGO_HIDX: st = bt_hidx(ctx_large, dg, out, &r.res.rounds); break;
-                    default: st = bt_msp(ctx_large, dg, out, &r.res.levels); break;
-                }
-                bt_graph_free(dg);
-            }
-            finish(rs, st);
-        }
-    }
 };
 
 extern "C" {
@@ -199,19 +186,27 @@ int bt_serve_start(const bt_serve_config* cfg, bt_server** out) {
     if (!s->cfg.large_edges) s->cfg.large_edges = 1000000;
     if (!s->cfg.max_batch) s->cfg.max_batch = 1024;
     if (!s->cfg.light_wedges) s->cfg.light_wedges = 2000000;
+    if (!s->cfg.batch_wedges) s->cfg.batch_wedges = 100000000;
     int st = bt_ctx_create(s->cfg.device, &s->ctx_light);
     if (st == BT_OK) st = bt_ctx_create(s->cfg.device, &s->ctx_heavy);
     if (st == BT_OK) st = bt_ctx_create(s->cfg.device, &s->ctx_large);
     if (st == BT_OK) {
-        // the light lane's peel on a quarter of the CUs, the others on the rest
+        // disjoint CU shares: a quarter for the light lane, three eighths
+        // for the heavy lane, the rest for large requests
         int const cus = s->ctx_light->cu_count;
-        int const light = std::max(1, cus / 4);
-        st = bt_ctx_set_option(s->ctx_light, BT_OPT_PEEL_BLOCKS, light);
-        if (st == BT_OK) st = bt_ctx_set_option(s->ctx_heavy, BT_OPT_PEEL_BLOCKS, cus - light);
-        if (st == BT_OK) st = bt_ctx_set_option(s->ctx_large, BT_OPT_PEEL_BLOCKS, cus - light);
+        int const light = s->cfg.light_cus ? int(s->cfg.light_cus) : std::max(1, cus / 4);
+        int const heavy = s->cfg.heavy_cus ? int(s->cfg.heavy_cus) : std::max(1, 3 * cus / 8);
+        int const large = s->cfg.large_cus ? int(s->cfg.large_cus) : std::max(1, cus - light - heavy);
+        if (light > cus || heavy > cus || large > cus) st = BT_ERR_INVALID;
+        // a share of all CUs is a cooperative launch on the whole device
+        auto share = [cus](int x) { return x >= cus ? 0 : x; };
+        if (st == BT_OK) st = bt_ctx_set_option(s->ctx_light, BT_OPT_PEEL_BLOCKS, share(light));
+        if (st == BT_OK) st = bt_ctx_set_option(s->ctx_heavy, BT_OPT_PEEL_BLOCKS, share(heavy));
+        if (st == BT_OK) st = bt_ctx_set_option(s->ctx_large, BT_OPT_PEEL_BLOCKS, share(large));
         // a served batch is one union on its lane's CUs
         if (st == BT_OK) st = bt_ctx_set_option(s->ctx_light, BT_OPT_BATCH_LANES, 1);
         if (st == BT_OK) st = bt_ctx_set_option(s->ctx_heavy, BT_OPT_BATCH_LANES, 1);
+        if (st == BT_OK) st = bt_ctx_set_option(s->ctx_large, BT_OPT_BATCH_LANES, 1);
     }
     auto destroy_ctx = [s] {
         if (s->ctx_light) bt_ctx_destroy(s->ctx_light);
@@ -226,7 +221,7 @@ int bt_serve_start(const bt_serve_config* cfg, bt_server** out) {
     try {
         s->thr_light = std::thread([s] { s->run_batches(s->light_q, s->ctx_light); });
         s->thr_heavy = std::thread([s] { s->run_batches(s->heavy_q, s->ctx_heavy); });
-        s->thr_large = std::thread([s] { s->run_large(); });
+        s->thr_large = std::thread([s] { s->run_batches(s->large_q, s->ctx_large); });
     } catch (...) {
         {
             std::lock_guard<std::mutex> lk(s->mu);
@@ -271,6 +266,7 @@ int bt_serve_submit(bt_server* s, const bt_host_graph* g, int algo, uint64_t* ti
     }
     r->algo = algo;
     r->est_bytes = estimate_bytes(*g, wedges);
+    r->wedges = wedges;
     if (r->est_bytes + kRunBytes > s->cfg.budget_bytes) return BT_ERR_BUDGET;
     r->t_submit = Clock::now();
     {

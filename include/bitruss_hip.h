@@ -195,12 +195,13 @@ int bt_batch_run(bt_ctx* ctx, bt_batch* b, int algo, uint64_t* const* outs, bt_b
  * No counterpart in the reference. A server owns three lanes on one device,
  * each a context and a worker thread. Small requests (fewer than
  * large_edges edges) are decomposed together: a lane's worker takes every
- * pending request of the oldest request's algorithm (up to max_batch) and
- * runs them as one batch, while requests that arrive meanwhile wait for the
- * next batch. Small requests with fewer than light_wedges priority wedges go
- * to the light lane, whose persistent peel occupies a quarter of the CUs;
- * the others to the heavy lane (the other CUs). Large requests run one at a
- * time on the plain path, sharing the heavy lane's CUs. A request is
+ * pending request of the oldest request's algorithm (up to max_batch; on
+ * the heavy lane up to batch_wedges priority wedges) and runs them as one
+ * batch, while requests that arrive meanwhile wait for the next batch. Small
+ * requests with fewer than light_wedges priority wedges go to the light
+ * lane, the others to the heavy lane; large requests are batched on a lane
+ * of their own. The lanes' persistent peels run on disjoint CU shares
+ * (light_cus, heavy_cus, large_cus), so no lane waits for another's. A request is
  * admitted when the estimated device bytes of the running requests plus its
  * own fit budget_bytes (the graph, ~140 B per edge, and its bloom index,
  * ~48 B per priority wedge); a request that alone exceeds the budget is
@@ -215,11 +216,18 @@ typedef struct bt_serve_config {
     uint32_t max_batch;     /* requests per batch at most (0: 1024)         */
     uint64_t light_wedges;  /* priority wedges below which a small request
                                takes the light lane (0: 2,000,000)          */
+    uint64_t batch_wedges;  /* priority wedges a heavy-lane batch takes at
+                               most, so that batches stay short (its first
+                               request always fits; 0: 100,000,000)         */
+    uint32_t light_cus;     /* CUs of the light lane's peels (0: 1/4)       */
+    uint32_t heavy_cus;     /* CUs of the heavy lane's peels (0: 3/8)       */
+    uint32_t large_cus;     /* CUs of the large lane's peels (0: the rest)  */
 } bt_serve_config;
 
 typedef struct bt_serve_stats {
     uint64_t submitted, completed, failed;
-    uint64_t batches, batched_requests, large_runs;
+    uint64_t batches, batched_requests;  /* light and heavy lanes        */
+    uint64_t large_runs;                 /* batches of the large lane      */
     uint64_t peak_bytes;    /* largest admitted estimate in flight          */
     uint64_t in_use_bytes;  /* admitted estimate in flight now              */
 } bt_serve_stats;

@@ -1,6 +1,7 @@
 """The serving loop (bt_serve_*) on the GPU: requests submitted while
-others run are batched, large ones run on the plain path, and every result
-equals the plain call on the same graph (alone or in the mix)."""
+others run are batched, large ones are batched on a lane of their own, and
+every result equals the plain call on the same graph (alone or in the
+mix)."""
 from __future__ import annotations
 
 import sys
@@ -46,14 +47,15 @@ def check(dev, g, algo, arr, scal):
 
 @pytest.mark.gpu
 def test_serve_mixed_stream(dev):
-    """Small requests arrive from another thread while batches run; two
-    requests are large (a low large_edges routes them to the plain path)."""
+    """Small requests arrive from another thread while batches run; three
+    requests are large (a low large_edges routes them to the large lane),
+    with two algorithms, so that the large lane runs at least two batches."""
     mix = [gb.Graph.generate("mix_graph", i) for i in range(48)]
     cut = 50000
     gs = [g for g in mix if g.m < cut]
     gs += [G.from_pairs(G.AW_EDGES), G.from_pairs(G.QUIRK7_EDGES), gb.Graph.from_edges([], [])]
-    large = [g for g in mix if g.m >= cut][:2]
-    assert len(large) == 2
+    large = [g for g in mix if g.m >= cut][:3]
+    assert len(large) == 3
     algos = ["msp", "hidx", "ebfc", "bfc"]
     with gb.Server(large_edges=cut, max_batch=16) as srv:
         tickets = []
@@ -66,7 +68,7 @@ def test_serve_mixed_stream(dev):
 
         th = threading.Thread(target=feed)
         th.start()
-        big = [(srv.submit(g, "msp"), g, "msp") for g in large]
+        big = [(srv.submit(g, a), g, a) for g, a in zip(large, ["msp", "hidx", "msp"])]
         th.join()
         results = [(srv.wait(t), g, a) for t, g, a in tickets + big]
         st = srv.stats()
