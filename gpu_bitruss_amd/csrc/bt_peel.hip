@@ -71,6 +71,7 @@ static_assert(kMB * kClaimUnroll == int(kClaimDummiesPerBlock), "dummy claim sta
 constexpr int kXCap = 2048;             // frontier edges buffered per wave
 constexpr int kBL = 1024;               // frontier edges per block claim batch
 constexpr uint32_t kBlkCap = 16384;     // claimed blooms a block keeps in its own list
+constexpr uint32_t kBigCap = 64;        // large blooms a block walks together per sub-round
 constexpr int kSp = 32;                 // u32 spacing of the barrier slots (128 B)
 constexpr uint64_t kInf = ~0ull;
 constexpr uint64_t kBarrierLimit = 30ull * 100000000ull;  // 30 s of the 100 MHz clock
@@ -821,6 +822,9 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
     __shared__ int s_ok;
     __shared__ uint32_t s_gbase;
     __shared__ unsigned s_pick;
+    __shared__ uint32_t s_big[kBigCap], s_big_bg[kBigCap];
+    __shared__ unsigned s_nbig, s_bcnt[4];
+    __shared__ uint64_t s_bc2[kC2Cap];
     uint32_t const G = gridDim.x, B = blockIdx.x, tid = threadIdx.x;
     int const wv = tid / kWave;
     uint32_t const gw = B * kWPB + wv, nw = G * kWPB;
@@ -939,7 +943,10 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
         // and first 64 middles of the next one load while the current bloom
         // is processed.
         uint32_t const nb_blk = nt > B ? (nt - B + G - 1) / G : 0u;
-        if (tid == 0) s_pick = kWPB;
+        if (tid == 0) {
+            s_pick = kWPB;
+            s_nbig = 0;
+        }
         __syncthreads();
         auto grab = [&]() -> uint32_t {
             uint32_t v = 0;
@@ -971,6 +978,20 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
             }
             uint32_t const kb = hdr & 0x7FFFFFFFu, up = hdr >> 31;
             if (kb < 2) continue;
+            // a bloom beyond the register tiers is walked by the whole block
+            // after this loop, unless the block's list of them is full
+            if (kb > uint32_t(16 * kWave)) {
+                uint32_t q = 0;
+                if (lane == 0) q = atomicAdd(&s_nbig, 1u);
+                q = __builtin_amdgcn_readfirstlane(q);
+                if (q < kBigCap) {
+                    if (lane == 0) {
+                        s_big[q] = s;
+                        s_big_bg[q] = bg;
+                    }
+                    continue;
+                }
+            }
             dec.L = level_of(bg);
             // a bloom frees at most two edges per middle
             xb.ensure(2 * (kb < uint32_t(16 * kWave) ? kb : uint32_t(16 * kWave)), X + 1, a, out);
@@ -979,6 +1000,108 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
             else if (kb <= uint32_t(4 * kWave)) done = process_bloom<4>(a, s, kb, up, r0, X, dec, c2, xb, out);
             else if (kb <= uint32_t(16 * kWave)) done = process_bloom<16>(a, s, kb, up, r0, X, dec, c2, xb, out);
             if (!done) process_large(a, s, kb, up, X, dec, c2, xb, out);
+        }
+        // ---- the large blooms, each by the whole block: the waves count
+        // the middle groups of interleaved 64-middle chunks, then apply the
+        // decrements; the both-frontier middles for the rule are gathered in
+        // LDS. A bloom whose both-frontier middles overflow that list goes
+        // to one wave (process_large: sorted lists). The bloom is compacted
+        // (by one wave) once at most half of it is left, so the walks stay
+        // proportional to its live middles.
+        __syncthreads();
+        uint32_t const nbig = s_nbig < kBigCap ? s_nbig : kBigCap;
+        for (uint32_t bi = 0; bi < nbig; bi++) {
+            uint32_t const s = s_big[bi];
+            uint32_t const hdr = ald(&a.hdr[s]);
+            uint32_t const kb = hdr & 0x7FFFFFFFu, up = hdr >> 31;
+            uint64_t* rec = a.rec + s;
+            dec.L = level_of(s_big_bg[bi]);
+            if (tid < 4) s_bcnt[tid] = 0;
+            __syncthreads();
+            uint32_t w0 = 0, w1 = 0, wp = 0;
+            for (uint32_t j0 = uint32_t(wv) * kWave; j0 < kb; j0 += kMB) {
+                uint32_t const j = j0 + lane;
+                int c = -1;
+                uint64_t r = 0;
+                if (j < kb) {
+                    r = ald(&rec[j]);
+                    c = classify(ald(&a.stamp[uint32_t(r >> 32)]), ald(&a.stamp[uint32_t(r)]), X);
+                }
+                uint64_t const b2 = __ballot(c == 2);
+                if (b2) {
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(&s_bcnt[2], uint32_t(__popcll(b2)));
+                    base = __builtin_amdgcn_readfirstlane(base);
+                    uint32_t const pos = base + __popcll(b2 & lanemask_lt());
+                    if (c == 2 && pos < uint32_t(kC2Cap)) s_bc2[pos] = r;
+                }
+                w0 += __popcll(__ballot(c == 0));
+                w1 += __popcll(__ballot(c == 1));
+                wp += __popcll(__ballot(c == 2 && uint32_t(r >> 32) < uint32_t(r)));
+            }
+            if (lane == 0) {
+                atomicAdd(&s_bcnt[0], w0);
+                atomicAdd(&s_bcnt[1], w1);
+                atomicAdd(&s_bcnt[3], wp);
+            }
+            __syncthreads();
+            uint32_t const n0 = s_bcnt[0], n1 = s_bcnt[1], n2 = s_bcnt[2];
+            if (n2 > a.c2cap && n1) {
+                if (wv == 0) process_large(a, s, kb, up, X, dec, c2, xb, out);
+                __syncthreads();
+                continue;
+            }
+            uint64_t const dec0 = uint64_t(n1) + n2;
+            for (uint32_t j0 = uint32_t(wv) * kWave; j0 < kb; j0 += kMB) {
+                xb.ensure(2 * kWave, X + 1, a, out);
+                uint32_t const j = j0 + lane;
+                int c = -1;
+                uint64_t r = 0;
+                uint32_t e1 = 0, e2 = 0, s1 = 0;
+                if (j < kb) {
+                    r = ald(&rec[j]);
+                    e1 = uint32_t(r >> 32);
+                    e2 = uint32_t(r);
+                    s1 = ald(&a.stamp[e1]);
+                    c = classify(s1, ald(&a.stamp[e2]), X);
+                }
+                bool x1 = false, x2 = false;
+                uint32_t f = e1;
+                if (c == 0 && dec0) {
+                    x1 = dec(e1, dec0);
+                    x2 = dec(e2, dec0);
+                } else if (c == 1) {
+                    bool const froot = s1 != X;
+                    f = froot ? e1 : e2;
+                    uint32_t const g = froot ? e2 : e1;
+                    uint64_t const d = uint64_t(n0) + (n1 - 1) + rule_sum(s_bc2, n2, froot, up, g);
+                    if (d) x1 = dec(f, d);
+                }
+                xb.put(x1, f);
+                xb.put(x2, e2);
+            }
+            __syncthreads();
+            if (n0 < 2) {
+                // no butterfly is left: the top claim stamp keeps the bloom
+                // out of every later claim list
+                if (tid == 0) afetch_max(&a.claim[s], ~0u);
+            } else if (2 * n0 <= kb && wv == 0) {
+                uint32_t wpos = 0;
+                for (uint32_t j0 = 0; j0 < kb; j0 += kWave) {
+                    uint32_t const j = j0 + lane;
+                    int c = -1;
+                    uint64_t r = 0;
+                    if (j < kb) {
+                        r = ald(&rec[j]);
+                        c = classify(ald(&a.stamp[uint32_t(r >> 32)]), ald(&a.stamp[uint32_t(r)]), X);
+                    }
+                    uint64_t const keepm = __ballot(c == 0);
+                    if (c == 0) ast(&rec[wpos + __popcll(keepm & lanemask_lt())], r);
+                    wpos += __popcll(keepm);
+                }
+                if (lane == 0) ast(&a.hdr[s], wpos | (up << 31));
+            }
+            __syncthreads();
         }
         tmin = dec.tmin;
         uint64_t const tw = wall_clock64() - tw0;
@@ -1411,9 +1534,20 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
             if (progress) {
                 double const sec =
                     std::chrono::duration<double>(std::chrono::steady_clock::now() - t_first).count();
-                std::fprintf(stderr, "[bt_peel] slice %u: level %llu, level steps %llu, sub-rounds %llu, %.1f s\n",
+                std::fprintf(stderr, "[bt_peel] slice %u: level %llu, level steps %llu, sub-rounds %llu, %.1f s",
                              slices, (unsigned long long)st[4], (unsigned long long)st[3],
                              (unsigned long long)st[2], sec);
+                if (a.diag) {
+                    // block 0's clock: phase time, its own work, the slowest waves
+                    MspCtl h{};
+                    BT_HIP_TRY(hipMemcpy(&h, ctl, sizeof(MspCtl), hipMemcpyDeviceToHost));
+                    std::fprintf(stderr,
+                                 "; collect %.1f ms (block 0 work %.1f), peel %.1f ms (block 0 work %.1f, "
+                                 "slowest waves %.1f), sorted %llu",
+                                 h.t_collect * 1e-5, h.t_work_collect * 1e-5, h.t_peel * 1e-5, h.t_work_peel * 1e-5,
+                                 h.pmax_sum * 1e-5, h.sorted);
+                }
+                std::fprintf(stderr, "\n");
                 std::fflush(stderr);
             }
             a.resume = 1;
