@@ -29,8 +29,9 @@
 //
 // Control: one cooperative launch, one block per CU; phases are separated
 // by a grid barrier, and after each barrier every block reads the same
-// per-block words and takes the same decision. Nothing in a phase is
-// written by many blocks to one word:
+// per-block words and takes the same decision. Hot words are not written
+// by many blocks (the two shared live-edge lists take one atomic per
+// 2,048-edge chunk):
 //   - the barrier: each block stores its arrival epoch in its own slot,
 //     block 0 watches all slots (one thread each) and releases with one
 //     store (about 1.6 us for 256 blocks; a shared counter serialises the
@@ -42,7 +43,10 @@
 //     supports) in private slots, reduced by every block at a level's end.
 // The blooms a frontier edge belongs to are claimed by the whole block at
 // the end of a phase (its incidence lists concatenated and walked by 512
-// lanes), so a hub edge's long list does not serialise one wave.
+// lanes), so a hub edge's long list does not serialise one wave; likewise a
+// bloom with more than 1,024 middles is walked by the whole block.
+// Time slices: the launch returns at the first level boundary after its
+// slice and is launched again to resume from the place it left in ctl.
 // Coherence across the 8 XCDs without cache-wide fences: every word that
 // one block writes and another reads later is accessed with relaxed
 // device-scope atomics, and a barrier first waits for the block's own
@@ -1160,7 +1164,7 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
                 return;
             }
             uint32_t const p = k & 1;
-            // ---- collect: frontier of level L from this block's live segment
+            // ---- collect: frontier of level L from the shared live list
             {
                 uint64_t const tp0 = wall_clock64();
                 if (B == 0 && tid == 0) ast(&ctl->ovf_n[(X + 1) % 3], 0u);
