@@ -76,6 +76,7 @@ constexpr int kXCap = 2048;             // frontier edges buffered per wave
 constexpr int kBL = 1024;               // frontier edges per block claim batch
 constexpr uint32_t kBlkCap = 16384;     // claimed blooms a block keeps in its own list
 constexpr uint32_t kBigCap = 64;        // large blooms a block walks together per sub-round
+constexpr uint32_t kWindowEdges = 4u << 20;  // live edges from which the collect uses a window
 constexpr int kSp = 32;                 // u32 spacing of the barrier slots (128 B)
 constexpr uint64_t kInf = ~0ull;
 constexpr uint64_t kBarrierLimit = 30ull * 100000000ull;  // 30 s of the 100 MHz clock
@@ -102,7 +103,9 @@ struct MspCtl {
     unsigned long long r_X, r_k, r_L, r_epoch;
     unsigned long long t_launch;     // block 0's clock at the launch
     unsigned long long sorted;       // bloom visits that sorted both-frontier lists
-    unsigned long long live_n[2];    // plain graphs: lengths of the two live-edge lists
+    unsigned long long live_n[2];    // plain graphs: lengths of the two collect lists
+    unsigned long long high_n[2];    // plain graphs: lengths of the two lists above the window
+    unsigned long long hp, W;        // current list above the window, the window's top
 };
 static_assert(sizeof(MspCtl) <= 128 * sizeof(uint64_t), "control block overlays dg->ctl");
 
@@ -147,6 +150,7 @@ struct MspArgs {
     uint32_t c2s_stride;
     int id_bits;
     uint32_t c2cap;        // both-frontier middles summed from the LDS list (<= kC2Cap)
+    uint32_t window_min;   // live edges from which a split opens a window
 };
 
 // relaxed device-scope accesses (coherent across XCDs)
@@ -446,6 +450,11 @@ __device__ __forceinline__ void claim_block(const MspArgs& a, uint32_t Y, const 
     }
 }
 
+// the next level's collect list and its length (plain graphs), for the
+// rare appends from Dec::settle; in LDS so that no register holds them
+__shared__ uint32_t* s_low_list;
+__shared__ unsigned long long* s_low_len;
+
 struct Dec {
     uint64_t* sup;
     uint32_t* stamp;
@@ -453,6 +462,9 @@ struct Dec {
     uint64_t L;
     uint32_t X;
     uint64_t tmin;  // smallest decremented support still above the level
+    // plain graphs: an edge whose support enters the collect window
+    // (<= W) joins the next level's collect list
+    uint64_t W;
     // subtracts d from the support of x (atomic, returns the old support)
     __device__ __forceinline__ uint64_t issue(uint32_t x, uint64_t d) const {
         return afetch_add(&sup[x], uint64_t(0) - d);
@@ -466,6 +478,10 @@ struct Dec {
             return true;
         }
         if (nw > L && nw < tmin) tmin = nw;
+        if (old > W && nw <= W) {
+            uint32_t const pos = uint32_t(afetch_add(s_low_len, 1ull));
+            ast(&s_low_list[pos], x);
+        }
         return false;
     }
     __device__ __forceinline__ bool operator()(uint32_t x, uint64_t d) { return settle(x, d, issue(x, d)); }
@@ -809,7 +825,10 @@ __device__ __forceinline__ void process_large(const MspArgs& a, uint32_t s, uint
     }
 }
 
-template <bool kBatch>
+// kWindow (plain graphs with many edges): the collect scans a support
+// window instead of every live edge; a separate instance keeps the code of
+// smaller graphs free of it (register pressure).
+template <bool kBatch, bool kWindow = false>
 __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
     __shared__ uint64_t s_gmin[kBatch ? kSpan : 1];
     __shared__ uint32_t s_part[kWPB + 1];
@@ -888,9 +907,10 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
 
     // ---- init: support-0 edges are final (process_support_zero); the rest
     // are live: in this block's segment of the live array (batches), or
-    // appended to live list 0 (plain graphs: every level's collect then
-    // spreads the list evenly over the blocks); the first level is their
-    // min support (slots of parity 1, so that level 0 can write parity 0)
+    // appended to the list above the window (plain graphs: the first level
+    // splits it; every level's collect spreads its list evenly over the
+    // blocks); the first level is their min support (slots of parity 1, so
+    // that level 0 can write parity 0)
     if (!a.resume) {
         uint32_t w = seg0;
         uint64_t mn = kInf;
@@ -913,9 +933,13 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
             if constexpr (kBatch) {
                 if (keep) a.live[w + ex] = i;
             } else {
-                if (tid == 0) s_gbase = tot ? uint32_t(afetch_add(&ctl->live_n[0], (unsigned long long)tot)) : 0u;
+                // with a window all go above it (the first level splits them),
+                // else to collect list 0
+                unsigned long long* const len = kWindow ? &ctl->high_n[0] : &ctl->live_n[0];
+                uint32_t* const list = kWindow ? a.live + 2 * size_t(a.m) : a.live;
+                if (tid == 0) s_gbase = tot ? uint32_t(afetch_add(len, (unsigned long long)tot)) : 0u;
                 __syncthreads();
-                if (keep) ast(&a.live[s_gbase + ex], i);  // read by other blocks
+                if (keep) ast(&list[s_gbase + ex], i);  // read by other blocks
                 __syncthreads();
             }
             w += tot;
@@ -925,6 +949,7 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
         if (!grid_sync(a, G, ++epoch, &s_ok)) return;
     }
     uint32_t X = a.resume ? uint32_t(ctl->r_X) : 2u, k = a.resume ? uint32_t(ctl->r_k) : 0u;
+    uint64_t W = a.resume ? ctl->W : 0ull;  // top of the collect window (plain graphs)
     uint32_t nt = 0;            // claimed blooms of sub-round X
     uint64_t t_col = 0, t_pe = 0, w_col = 0, w_pe = 0, pmax_sum = 0;
 
@@ -935,7 +960,11 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
         uint64_t const tp0 = wall_clock64();
         if (B == 0 && tid == 0) ast(&ctl->ovf_n[(X + 2) % 3], 0u);
         Out const out = out_for(X + 1);
-        Dec dec{a.sup, a.stamp, a.res, 0, X, tmin};
+        Dec dec{a.sup, a.stamp, a.res, 0, X, tmin, kWindow ? W : kInf};
+        if (kWindow && tid == 0) {
+            s_low_list = a.live + size_t(p ^ 1u) * a.m;
+            s_low_len = &ctl->live_n[p ^ 1u];
+        }
         uint64_t* c2 = s_c2[wv];
         CrossBuf xb{s_xl[wv], 0, ws};
         int const lane = tid & (kWave - 1);
@@ -1145,7 +1174,7 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
 
     if constexpr (!kBatch) {
         uint64_t L = a.resume ? ctl->r_L : level_min(1);
-        while (L != kInf) {
+        for (;;) {
             // the slice is over: `yield` was set before the last barrier and
             // is rewritten only after the next one, so every block agrees
             if (a.slice_ticks && ald(&ctl->yield)) {
@@ -1164,6 +1193,89 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
                 return;
             }
             uint32_t const p = k & 1;
+            // ---- window: the collect scans only the live edges with support
+            // <= W (list p); the others wait in the list above the window,
+            // and a decrement that takes an edge into the window appends it
+            // to the next level's list. When list p is empty, every live
+            // edge is above the window (a live edge that entered it is in
+            // list p until it is peeled), so the level is not known from the
+            // slots: the minimum of the list above is taken (phase 1), and
+            // the list is split at the new window top (phase 2), dropping
+            // the edges already peeled. The same when the slots hold no
+            // level (list p's edges are all peeled). A level taken from the
+            // slots while list p has no live edge can exceed W: its collect
+            // then finds nothing, changes nothing, and the next level splits.
+            if (!kWindow && L == kInf) break;
+            if (kWindow && (ald(&ctl->live_n[p]) == 0 || L == kInf)) {
+                uint32_t const hp = uint32_t(ald(&ctl->hp));
+                const uint32_t* hcur = a.live + size_t(2 + hp) * a.m;
+                uint32_t* hnxt = a.live + size_t(3 - hp) * a.m;
+                uint32_t* lcur = a.live + size_t(p) * a.m;
+                uint32_t const hcnt = uint32_t(ald(&ctl->high_n[hp]));
+                constexpr int U = 4;
+                uint64_t mn = kInf;
+                for (uint32_t base = B * (kMB * U); base < hcnt; base += G * (kMB * U)) {
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        uint32_t const i = base + u * kMB + tid;
+                        if (i < hcnt) {
+                            uint32_t const e = ald(&hcur[i]);
+                            if (ald(&a.stamp[e]) == 0) {
+                                uint64_t const v = ald(&a.sup[e]);
+                                mn = v < mn ? v : mn;
+                            }
+                        }
+                    }
+                }
+                set_slots(p, block_min_all(mn, s_red), kInf);
+                if (!grid_sync(a, G, ++epoch, &s_ok)) return;
+                L = level_min(p);
+                if (L == kInf) break;
+                // a window pays on many live edges; below that, one list
+                // holds them all (no edge is above the window)
+                W = hcnt > a.window_min ? L + (L / 4 > 16 ? L / 4 : 16) : kInf;
+                for (uint32_t base = B * (kMB * U); base < hcnt; base += G * (kMB * U)) {
+                    uint32_t e[U];
+                    int where[U];  // 0 dropped, 1 window, 2 above
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        uint32_t const i = base + u * kMB + tid;
+                        e[u] = i < hcnt ? ald(&hcur[i]) : 0u;
+                        where[u] = 0;
+                        if (i < hcnt && ald(&a.stamp[e[u]]) == 0) where[u] = ald(&a.sup[e[u]]) <= W ? 1 : 2;
+                    }
+                    uint32_t n1 = 0, n2 = 0;
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        n1 += where[u] == 1 ? 1u : 0u;
+                        n2 += where[u] == 2 ? 1u : 0u;
+                    }
+                    uint32_t t1 = 0, t2 = 0;
+                    uint32_t const x1 = block_excl_scan<kMB>(n1, s_part, t1);
+                    uint32_t const x2 = block_excl_scan<kMB>(n2, s_part, t2);
+                    if (tid == 0) {
+                        s_bcnt[0] = t1 ? uint32_t(afetch_add(&ctl->live_n[p], (unsigned long long)t1)) : 0u;
+                        s_bcnt[1] = t2 ? uint32_t(afetch_add(&ctl->high_n[hp ^ 1u], (unsigned long long)t2)) : 0u;
+                    }
+                    __syncthreads();
+                    uint32_t q1 = s_bcnt[0] + x1, q2 = s_bcnt[1] + x2;
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        if (where[u] == 1) ast(&lcur[q1++], e[u]);
+                        if (where[u] == 2) ast(&hnxt[q2++], e[u]);
+                    }
+                    __syncthreads();
+                }
+                if (!grid_sync(a, G, ++epoch, &s_ok)) return;
+                // every block has read the old list's length and the window
+                // is the same in every block; block 0 records them for the
+                // next split and for a resumed launch
+                if (B == 0 && tid == 0) {
+                    ast(&ctl->high_n[hp], 0ull);
+                    ast(&ctl->hp, hp ^ 1u);
+                    ast(&ctl->W, W);
+                }
+            }
             // ---- collect: frontier of level L from the shared live list
             {
                 uint64_t const tp0 = wall_clock64();
@@ -1429,8 +1541,15 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     if (G == 0 || G >= unsigned(kMB)) return BT_ERR_DEVICE;
     int per_cu = 0;
     bool const batch = dg->egraph != nullptr && dg->ngraph > 0 && c->batch_levels == BT_BATCH_LEVELS_PER_GRAPH;
-    const void* kern = batch ? reinterpret_cast<const void*>(k_msp_persist<true>)
-                             : reinterpret_cast<const void*>(k_msp_persist<false>);
+    // the window for plain graphs with many edges (the live edges at the
+    // start are at most m); BT_PEEL_WINDOW_EDGES lowers the threshold, to
+    // exercise the window on small graphs
+    uint64_t window_edges = kWindowEdges;
+    if (const char* e = getenv("BT_PEEL_WINDOW_EDGES")) window_edges = uint64_t(std::max(0L, atol(e)));
+    bool const window = !batch && m > window_edges;
+    const void* kern = batch    ? reinterpret_cast<const void*>(k_msp_persist<true>)
+                       : window ? reinterpret_cast<const void*>(k_msp_persist<false, true>)
+                                : reinterpret_cast<const void*>(k_msp_persist<false>);
     BT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kMB, 0));
     if (per_cu < 1) return BT_ERR_DEVICE;
     uint32_t const nbloom = uint32_t(std::max<uint64_t>(ix.NB, 1));
@@ -1446,7 +1565,8 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     if (c2s_stride) BT_TRY(bt_buf_reserve(ix.c2s, sizeof(uint32_t) * size_t(G) * kWPB * c2s_stride));
     // claim stamps: R real ones, then the dummies (both sized by the index build)
     if (ix.bclaim.bytes < sizeof(uint32_t) * (ix.R + size_t(G) * kClaimDummiesPerBlock)) return BT_ERR_INTERNAL;
-    BT_TRY(bt_buf_reserve(ix.live, sizeof(uint32_t) * 2 * std::max<size_t>(m, 1)));
+    // plain graphs: two collect lists and two lists above the window
+    BT_TRY(bt_buf_reserve(ix.live, sizeof(uint32_t) * 4 * std::max<size_t>(m, 1)));
     BT_TRY(bt_buf_reserve(ix.area, sizeof(uint32_t) * 3 * size_t(G) * kBlkCap));
     BT_TRY(bt_buf_reserve(ix.touched, sizeof(uint32_t) * 3 * size_t(nbloom)));
     // per-block words: live counts, list lengths, level slots, barrier slots
@@ -1484,6 +1604,7 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     a.c2s = c2s_stride ? static_cast<uint32_t*>(ix.c2s.ptr) : nullptr;
     a.c2s_stride = c2s_stride;
     a.c2cap = c2cap;
+    a.window_min = uint32_t(std::min<uint64_t>(window_edges / 4, 0xFFFFFFFFull));
     a.id_bits = 1;
     while (a.id_bits < 32 && ((m ? m - 1 : 0) >> a.id_bits)) a.id_bits++;
     if (batch) {

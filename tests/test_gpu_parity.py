@@ -247,6 +247,34 @@ def test_sorted_rule_lists(dev, monkeypatch):
 
 
 @pytest.mark.gpu
+def test_msp_collect_window(dev, monkeypatch):
+    """The collect window (a list of the live edges with support <= W, the
+    others above it, split again when the window empties), which the peel
+    uses on graphs with millions of edges, forced onto smaller graphs: the
+    oracle's results and the 2M-edge golden."""
+    monkeypatch.setenv("BT_PEEL_WINDOW_EDGES", "0")
+    for name, g in [("quirk%d" % s, G.quirk_graph(s)) for s in G.QUIRK_SEEDS[:3]] + \
+            [("mix2", gb.Graph.generate("mix_graph", 2)), ("hub", G.hub_graph(5, 6000, 20000, 400, 7000))]:
+        say(f"window: {name} m={g.m}")
+        dg = dev.upload(g)
+        res, levels = dg.msp()
+        dg.free()
+        ores, olevels, _ = oracle.msp(g, 8)
+        assert np.array_equal(res, ores) and levels == olevels, name
+    gold = json.load(open(os.path.join(GOLDEN, "powerlaw_2m.json")))
+    d = gb.Device(dev.device)
+    try:
+        d.set_peel_slice_ms(5)  # resumes must keep the window's state
+        dg = d.upload(gb.Graph.generate("powerlaw_2m"))
+        say("window: powerlaw_2m in 5 ms slices")
+        res, levels = dg.msp()
+        dg.free()
+    finally:
+        d.close()
+    assert digest(res) == gold["msp_sha256"] and levels == gold["msp_levels"]
+
+
+@pytest.mark.gpu
 def test_peel_on_part_of_the_device(dev):
     """The persistent peel on a subset of the CUs gives the same results,
     also with two contexts peeling side by side on disjoint CU counts."""
