@@ -151,6 +151,7 @@ struct MspArgs {
     int id_bits;
     uint32_t c2cap;        // both-frontier middles summed from the LDS list (<= kC2Cap)
     uint32_t window_min;   // live edges from which a split opens a window
+    uint64_t max_level;    // plain graphs: stop after this level (0: never)
 };
 
 // relaxed device-scope accesses (coherent across XCDs)
@@ -1276,6 +1277,8 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
                     ast(&ctl->W, W);
                 }
             }
+            // a prefix of the decomposition only: levels up to max_level
+            if (a.max_level && L > a.max_level) break;
             // ---- collect: frontier of level L from the shared live list
             {
                 uint64_t const tp0 = wall_clock64();
@@ -1513,6 +1516,14 @@ __global__ void k_msp_init(MspCtl* ctl, uint32_t* arrive, uint32_t n_arrive, uin
     }
 }
 
+// after a prefix of the decomposition: the edges not peeled yet report their
+// current support
+__global__ void k_prefix_state(uint32_t m, const uint32_t* __restrict__ stamp, const uint64_t* __restrict__ sup,
+                               uint64_t* __restrict__ res) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
+        if (stamp[i] == 0) res[i] = sup[i];
+}
+
 __global__ void k_max_res(uint32_t m, const uint64_t* __restrict__ a, uint64_t* __restrict__ out) {
     uint64_t mx = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
@@ -1605,6 +1616,10 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     a.c2s_stride = c2s_stride;
     a.c2cap = c2cap;
     a.window_min = uint32_t(std::min<uint64_t>(window_edges / 4, 0xFFFFFFFFull));
+    // BT_PEEL_MAX_LEVEL: stop after that level and report the state of the
+    // prefix (peeled edges their result, the others their current support),
+    // to check a prefix of a decomposition too long to run whole
+    if (const char* e = getenv("BT_PEEL_MAX_LEVEL")) a.max_level = batch ? 0 : uint64_t(std::max(0LL, atoll(e)));
     a.id_bits = 1;
     while (a.id_bits < 32 && ((m ? m - 1 : 0) >> a.id_bits)) a.id_bits++;
     if (batch) {
@@ -1681,6 +1696,8 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
         BT_TRY(tm.start(7));
         BT_HIP_TRY(hipStreamSynchronize(s));
     }
+    if (a.max_level && m)
+        k_prefix_state<<<grid_for(m, 256, unsigned(c->cu_count) * 8), 256, 0, s>>>(m, a.stamp, a.sup, dg->res);
     // the level count the reference logs is the largest result
     BT_HIP_TRY(hipMemsetAsync(&ctl->level_max, 0, sizeof(uint64_t), s));
     if (m)

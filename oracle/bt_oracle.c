@@ -326,10 +326,12 @@ static inline void attribute(peel_ctx* c, uint32_t uv, uint32_t wv, uint32_t wx,
 }
 
 /* max_seconds > 0 stops after the first level that ends past that time
- * (a bounded sample of the decomposition, for timing); *complete says
- * whether every edge was peeled. */
+ * (a bounded sample of the decomposition, for timing); max_level > 0 stops
+ * after that level (the state of a prefix of the decomposition: peeled
+ * edges hold their result, the others their current support); *complete
+ * says whether every edge was peeled. */
 static int msp_body(const or_graph* g, int threads, uint64_t* bitruss, uint32_t* levels_out,
-                    or_stats* stats, double max_seconds, int* complete) {
+                    or_stats* stats, double max_seconds, uint64_t max_level, int* complete) {
     if (check_graph(g) || (g->m && !bitruss)) return OR_EINVAL;
     const double t_start = omp_get_wtime();
     const uint32_t m = g->m, n = g->n;
@@ -454,6 +456,7 @@ static int msp_body(const or_graph* g, int threads, uint64_t* bitruss, uint32_t*
         }
         level++;
         if (max_seconds > 0 && omp_get_wtime() - t_start > max_seconds) break;
+        if (max_level > 0 && level > max_level) break;
     }
     if (complete) *complete = edge_num <= 0;
     free(processed); free(in_curr); free(in_next); free(curr); free(next);
@@ -472,12 +475,18 @@ static int msp_body(const or_graph* g, int threads, uint64_t* bitruss, uint32_t*
 
 int or_msp(const or_graph* g, int threads, uint64_t* bitruss, uint32_t* levels_out,
            or_stats* stats) {
-    return msp_body(g, threads, bitruss, levels_out, stats, 0.0, NULL);
+    return msp_body(g, threads, bitruss, levels_out, stats, 0.0, 0, NULL);
 }
 
 int or_msp_sample(const or_graph* g, int threads, double max_seconds, uint64_t* work,
                   uint32_t* levels_done, int* complete, or_stats* stats) {
-    return msp_body(g, threads, work, levels_done, stats, max_seconds, complete);
+    return msp_body(g, threads, work, levels_done, stats, max_seconds, 0, complete);
+}
+
+int or_msp_levels(const or_graph* g, int threads, uint64_t max_level, uint64_t* state, uint32_t* levels_done,
+                  int* complete, or_stats* stats) {
+    if (max_level == 0) return OR_EINVAL;
+    return msp_body(g, threads, state, levels_done, stats, 0.0, max_level, complete);
 }
 
 /* ------------------------------------------------------- textbook peel */

@@ -58,6 +58,11 @@ int or_msp(const or_graph* g, int threads, uint64_t* bitruss, uint32_t* levels,
  * its result. The stats count the work done. */
 int or_msp_sample(const or_graph* g, int threads, double max_seconds, uint64_t* work,
                   uint32_t* levels_done, int* complete, or_stats* stats);
+/* msp through level max_level only: state[e] is the result of a peeled
+ * edge and the current support of the others (cpu_bitruss.cpp:536-597 with
+ * the level loop cut) */
+int or_msp_levels(const or_graph* g, int threads, uint64_t max_level, uint64_t* state, uint32_t* levels_done,
+                  int* complete, or_stats* stats);
 int or_peel_seq(const or_graph* g, uint64_t* bitruss);
 
 #ifdef __cplusplus

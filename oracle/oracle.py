@@ -57,8 +57,10 @@ def lib() -> C.CDLL:
         L.or_msp.argtypes = [C.POINTER(OrGraph), C.c_int, u64p, C.POINTER(C.c_uint32), C.POINTER(OrStats)]
         L.or_msp_sample.argtypes = [C.POINTER(OrGraph), C.c_int, C.c_double, u64p, C.POINTER(C.c_uint32),
                                     C.POINTER(C.c_int), C.POINTER(OrStats)]
+        L.or_msp_levels.argtypes = [C.POINTER(OrGraph), C.c_int, C.c_uint64, u64p, C.POINTER(C.c_uint32),
+                                    C.POINTER(C.c_int), C.POINTER(OrStats)]
         L.or_peel_seq.argtypes = [C.POINTER(OrGraph), u64p]
-        for f in (L.or_ebfc, L.or_bfc, L.or_hidx, L.or_msp, L.or_msp_sample, L.or_peel_seq):
+        for f in (L.or_ebfc, L.or_bfc, L.or_hidx, L.or_msp, L.or_msp_sample, L.or_msp_levels, L.or_peel_seq):
             f.restype = C.c_int
         _lib = L
     return _lib
@@ -125,6 +127,17 @@ def msp_sample(g, threads: int, max_seconds: float):
     _run("msp_sample", lib().or_msp_sample(C.byref(v.s), threads, max_seconds, work.ctypes.data_as(u64p),
                                            C.byref(levels), C.byref(done), C.byref(st)))
     return levels.value, bool(done.value), st.as_dict()
+
+
+def msp_levels(g, threads: int, max_level: int):
+    """msp through level max_level only: (state[m] uint64 — the result of a
+    peeled edge, the current support of the others —, complete, stats)."""
+    v = _View(g)
+    out = np.zeros(max(g.m, 1), np.uint64)
+    levels, done, st = C.c_uint32(), C.c_int(), OrStats()
+    _run("msp_levels", lib().or_msp_levels(C.byref(v.s), threads, max_level, out.ctypes.data_as(u64p),
+                                           C.byref(levels), C.byref(done), C.byref(st)))
+    return out[: g.m], bool(done.value), st.as_dict()
 
 
 def peel_seq(g):

@@ -21,6 +21,12 @@ the oracle CLI made separately with --hidx-from:
 
     python tests/golden/make_golden.py skewed_60m --skip-msp --skip-hidx \
         --msp-from c3_msp.json c3_msp.u64
+
+A decomposition too long for the oracle is recorded by a prefix instead:
+--msp-prefix L stores the state after levels 1..L (peeled edges their
+result, the others their current support):
+
+    python tests/golden/make_golden.py skewed_60m --skip-msp --skip-hidx --msp-prefix 1000
 """
 from __future__ import annotations
 
@@ -51,6 +57,8 @@ def main() -> None:
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 4)
     ap.add_argument("--skip-hidx", action="store_true")
     ap.add_argument("--skip-msp", action="store_true")
+    ap.add_argument("--msp-prefix", type=int, default=0, metavar="L",
+                    help="also record the msp state after levels 1..L")
     ap.add_argument("--msp-from", nargs=2, metavar=("JSON", "U64"),
                     help="oracle CLI output (JSON line, --dump array) of msp on the one workload")
     ap.add_argument("--hidx-from", nargs=2, metavar=("JSON", "U64"),
@@ -70,6 +78,10 @@ def main() -> None:
         if not args.skip_msp:
             res, levels, st2 = oracle.msp(g, args.threads)
             out.update({"msp_levels": levels, "msp_sha256": digest(res), "msp_stats": st2})
+        if args.msp_prefix:
+            st, _, st4 = oracle.msp_levels(g, args.threads, args.msp_prefix)
+            out.update({"msp_prefix_levels": args.msp_prefix, "msp_prefix_sha256": digest(st),
+                        "msp_prefix_stats": {k: st4[k] for k in ("w_peel", "b_peel", "sum_curr", "sub_rounds")}})
         if args.msp_from:
             with open(args.msp_from[0]) as f:
                 run = json.loads(f.read().strip().splitlines()[-1])

@@ -275,6 +275,27 @@ def test_msp_collect_window(dev, monkeypatch):
 
 
 @pytest.mark.gpu
+def test_msp_level_prefix(dev, monkeypatch):
+    """A prefix of the decomposition (levels up to a bound; the peeled edges
+    hold their result, the others their current support) equals the
+    oracle's state after the same levels, with and without the collect
+    window."""
+    graphs = [("quirk%d" % s, G.quirk_graph(s)) for s in G.QUIRK_SEEDS[:3]]
+    graphs += [("mix2", gb.Graph.generate("mix_graph", 2)), ("hub", G.hub_graph(5, 6000, 20000, 400, 7000))]
+    for window in ("0", "100000000"):
+        monkeypatch.setenv("BT_PEEL_WINDOW_EDGES", window)
+        for bound in (1, 3, 40):
+            monkeypatch.setenv("BT_PEEL_MAX_LEVEL", str(bound))
+            for name, g in graphs:
+                dg = dev.upload(g)
+                res, _ = dg.msp()
+                dg.free()
+                st, _, _ = oracle.msp_levels(g, 8, bound)
+                assert np.array_equal(res, st), (name, bound, window)
+        say(f"level prefixes equal the oracle's (window threshold {window})")
+
+
+@pytest.mark.gpu
 def test_peel_on_part_of_the_device(dev):
     """The persistent peel on a subset of the CUs gives the same results,
     also with two contexts peeling side by side on disjoint CU counts."""
@@ -332,3 +353,18 @@ def test_skewed_60m_golden(dev):
     assert digest(sup) == gold["ebfc_sha256"]
     assert dev.last_timings()["wedges_count"] == gold["w_cnt"]
     dg.free()
+
+
+@pytest.mark.gpu
+def test_skewed_60m_msp_prefix(dev, monkeypatch):
+    """The 60M-edge graph's msp decomposition through its first levels
+    (the whole one runs for hours): the state after them — the peeled edges'
+    results and the others' current supports — against the oracle's."""
+    gold = json.load(open(os.path.join(GOLDEN, "skewed_60m.json")))
+    monkeypatch.setenv("BT_PEEL_MAX_LEVEL", str(gold["msp_prefix_levels"]))
+    g = gb.Graph.generate("skewed_60m")
+    dg = dev.upload(g)
+    say(f"skewed_60m msp through level {gold['msp_prefix_levels']}")
+    res, _ = dg.msp()
+    dg.free()
+    assert digest(res) == gold["msp_prefix_sha256"]
