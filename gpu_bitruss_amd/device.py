@@ -232,9 +232,10 @@ class Batch:
 
 
 class Server:
-    """Request loop with continuous batching (bt_serve_*): small graphs are
-    decomposed together in batches, large ones one at a time, under a
-    device-memory admission budget."""
+    """Request loop with continuous batching (bt_serve_*): light, heavy and
+    large requests are batched on three lanes with disjoint CU shares
+    (`cus` = light, heavy, large; 0 for the defaults), under a device-memory
+    admission budget."""
 
     def __init__(self, device: int = 0, budget_bytes: int = 0, large_edges: int = 0, max_batch: int = 0,
                  light_wedges: int = 0, batch_wedges: int = 0, cus: tuple = (0, 0, 0)):
