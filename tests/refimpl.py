@@ -92,12 +92,14 @@ def textbook_peel(g):
     return out
 
 
-def msp_by_butterflies(g, variant: str = ""):
+def msp_by_butterflies(g, variant: str = "", max_level: int = 0):
     """(result list, levels) of the reference msp, butterfly-centric.
 
     `variant` plants a deliberate mistake for the tests that show the pins
     can see one: "always" decrements every single survivor, "swap_pq"
-    exchanges the roles of the two frontier edges adjacent to it."""
+    exchanges the roles of the two frontier edges adjacent to it.
+    `max_level` > 0 stops before the first level above it: the list then
+    holds the peeled edges' results and the other edges' supports."""
     _, sup = brute_support(g)
     sup = list(sup)
     bfs = butterflies(g)
@@ -108,6 +110,8 @@ def msp_by_butterflies(g, variant: str = ""):
         if not alive:
             break
         level = min(sup[e] for e in alive)
+        if max_level and level > max_level:
+            break
         level_max = level
         curr = {e for e in alive if sup[e] == level}
         while curr:

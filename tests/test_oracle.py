@@ -101,6 +101,22 @@ def test_msp_rule_on_quirk_graphs(seed):
         assert got.tolist() != refimpl.textbook_peel(g)
 
 
+@pytest.mark.parametrize("seed", G.QUIRK_SEEDS[:4] + [0, 1])
+def test_msp_level_prefix_matches_butterfly_centric_rule(seed):
+    """or_msp_levels (the state after levels 1..L) against the
+    butterfly-centric restatement stopped at the same level, and the whole
+    decomposition once L reaches the last level."""
+    g = G.quirk_graph(seed) if seed in G.QUIRK_SEEDS else G.random_bipartite(
+        500 + seed, 10, 9, 0.6)
+    full, _ = refimpl.msp_by_butterflies(g)
+    top = max(full) if full else 0
+    for bound in sorted({1, 2, 3, max(1, top // 2), top, top + 3}):
+        want, _ = refimpl.msp_by_butterflies(g, max_level=bound)
+        got, complete, _ = oracle.msp_levels(g, 2, bound)
+        assert got.tolist() == want, bound
+        assert complete == (bound >= top)
+
+
 @pytest.mark.parametrize("variant", ["always", "swap_pq"])
 def test_planted_mistake_is_caught(variant):
     """A restatement with a plausible slip in the single-survivor rule must
