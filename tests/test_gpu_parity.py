@@ -361,6 +361,8 @@ def test_skewed_60m_msp_prefix(dev, monkeypatch):
     (the whole one runs for hours): the state after them — the peeled edges'
     results and the others' current supports — against the oracle's."""
     gold = json.load(open(os.path.join(GOLDEN, "skewed_60m.json")))
+    if "msp_prefix_sha256" not in gold:
+        pytest.skip("the oracle's prefix state of skewed_60m is not recorded (make_golden.py --msp-prefix)")
     monkeypatch.setenv("BT_PEEL_MAX_LEVEL", str(gold["msp_prefix_levels"]))
     g = gb.Graph.generate("skewed_60m")
     dg = dev.upload(g)
