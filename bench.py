@@ -125,6 +125,23 @@ def bench_mix(args) -> int:
     b.free()
     dev.close()
     t = float(np.mean(tb))
+    cpu = None
+    if not args.no_cpu:
+        # the oracle's whole msp on the mix graphs in order, until the
+        # time bound: graphs/s on the host
+        from oracle import oracle
+
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline: oracle msp on the mix graphs in order, {threads} threads, {args.cpu_seconds:.0f} s")
+        t0 = time.perf_counter()
+        done = 0
+        while done < k and time.perf_counter() - t0 < args.cpu_seconds:
+            oracle.msp(gs[done], threads)
+            done += 1
+        secs = time.perf_counter() - t0
+        cpu = {"value": done / secs, "unit": "graphs/s", "cores": threads, "kind": "port",
+               "sample": f"mix graphs 0..{done - 1} ({sum(g.m for g in gs[:done])} edges), whole msp each, "
+                         f"{secs:.1f} s on {threads} threads"}
     print(json.dumps({
         "metric": "graphs/s, batched msp vs plain (4,096-graph mix)", "value": k / t, "unit": "graphs/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * t,
@@ -133,6 +150,7 @@ def bench_mix(args) -> int:
                                "[50, 5000], density log-uniform in [0.5%, 5%], even seeds uniform, odd Chung-Lu 2.2",
                    "m": m_total, "algo": "msp", "lanes": args.lanes or 2},
         "plain_graphs_per_s": k / tp, "plain_s": tp, "speedup": tp / t, "identical": same,
+        "cpu_baseline": cpu,
     }), flush=True)
     return 0
 
