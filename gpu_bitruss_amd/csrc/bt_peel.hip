@@ -152,6 +152,7 @@ struct MspArgs {
     uint32_t c2cap;        // both-frontier middles summed from the LDS list (<= kC2Cap)
     uint32_t window_min;   // live edges from which a split opens a window
     uint64_t max_level;    // plain graphs: stop after this level (0: never)
+    uint32_t big_min;      // blooms with more middles are walked by the whole block
 };
 
 // relaxed device-scope accesses (coherent across XCDs)
@@ -1012,9 +1013,9 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
             }
             uint32_t const kb = hdr & 0x7FFFFFFFu, up = hdr >> 31;
             if (kb < 2) continue;
-            // a bloom beyond the register tiers is walked by the whole block
-            // after this loop, unless the block's list of them is full
-            if (kb > uint32_t(16 * kWave)) {
+            // a large bloom is walked by the whole block after this loop,
+            // unless the block's list of them is full
+            if (kb > a.big_min) {
                 uint32_t q = 0;
                 if (lane == 0) q = atomicAdd(&s_nbig, 1u);
                 q = __builtin_amdgcn_readfirstlane(q);
@@ -1616,6 +1617,9 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     a.c2s_stride = c2s_stride;
     a.c2cap = c2cap;
     a.window_min = uint32_t(std::min<uint64_t>(window_edges / 4, 0xFFFFFFFFull));
+    // blooms beyond the register tiers (BT_PEEL_BIG_MIDDLES: another bound)
+    a.big_min = 16 * kWave;
+    if (const char* e = getenv("BT_PEEL_BIG_MIDDLES")) a.big_min = uint32_t(std::max(1L, atol(e)));
     // BT_PEEL_MAX_LEVEL: stop after that level and report the state of the
     // prefix (peeled edges their result, the others their current support),
     // to check a prefix of a decomposition too long to run whole
