@@ -113,6 +113,14 @@ BATCH_LEVELS = {"per_graph": 0, "union": 1}
 OPT_PEEL_BLOCKS = 5
 OPT_PEEL_SLICE_MS = 6
 OPT_BATCH_LANES = 7
+OPT_PEEL_MAX_LEVEL = 8
+OPT_PEEL_C2CAP = 9
+OPT_PEEL_WINDOW_EDGES = 10
+OPT_PEEL_BIG_MIDDLES = 11
+# statuses (enum bt_status, include/bt_graph.h) the wrappers act on
+BT_PARTIAL = 10
+BT_ERR_BUSY = 11
+BT_ERR_SCRATCH = 12
 MSP_PATHS = {"auto": 0, "merge": 1, "index": 2}
 ALGOS = {"bfc": 1, "ebfc": 2, "hidx": 3, "msp": 4}
 

@@ -306,7 +306,11 @@ int bt_batch_run(bt_ctx* ctx, bt_batch* b, int algo, uint64_t* const* outs, bt_b
     }
     // lane contexts: each persistent peel takes its share of the CUs
     int const cus = ctx->peel_blocks > 0 && ctx->peel_blocks < ctx->cu_count ? ctx->peel_blocks : ctx->cu_count;
-    int const share = std::max(1, cus / nl);
+    // shares are whole multiples of the 8 XCDs: the blocks of a launch go
+    // round-robin over the XCDs, so a multiple of 8 takes the same number
+    // of CUs on each, and shares that add up to at most the CU count fit
+    // every XCD together
+    int const share = std::max(kXcds, cus / nl / kXcds * kXcds);
     if (b->lane_ctx.empty()) {
         for (int l = 0; l < nl; l++) {
             bt_ctx* lc = nullptr;
@@ -320,6 +324,9 @@ int bt_batch_run(bt_ctx* ctx, bt_batch* b, int algo, uint64_t* const* outs, bt_b
         lc->index_mem_frac = ctx->index_mem_frac;
         lc->batch_levels = ctx->batch_levels;
         lc->peel_slice_ms = ctx->peel_slice_ms;
+        lc->peel_c2cap = ctx->peel_c2cap;
+        lc->peel_window_edges = ctx->peel_window_edges;
+        lc->peel_big_middles = ctx->peel_big_middles;
         lc->peel_blocks = share < lc->cu_count ? share : 0;
     }
     std::vector<int> status(nl, BT_OK);

@@ -335,7 +335,19 @@ int bt_msp_impl(bt_dgraph* dg, uint32_t* levels_out) {
     bt_ctx* c = dg->ctx;
     bool const want_index = c->coop && c->msp_path != BT_MSP_PATH_MERGE;
     BT_TRY(bt_count_impl(dg, true, nullptr, nullptr, want_index));
-    if (want_index && c->ix.R != kNoIndex) return bt_msp_index_peel(dg, levels_out);
+    if (want_index && c->ix.R != kNoIndex) {
+        int st = bt_msp_index_peel(dg, levels_out);
+        if (st != BT_ERR_BUSY) return st;
+        // the peel's share of the CUs was held by other work (its grid
+        // barrier timed out): once more, from the supports, as one
+        // cooperative launch on the whole device
+        int const share = c->peel_blocks;
+        c->peel_blocks = 0;
+        st = bt_count_impl(dg, true, nullptr, nullptr, true);
+        if (st == BT_OK) st = c->ix.R != kNoIndex ? bt_msp_index_peel(dg, levels_out) : BT_ERR_BUDGET;
+        c->peel_blocks = share;
+        return st == BT_ERR_INTERNAL ? BT_ERR_BUSY : st;
+    }
     if (c->msp_path == BT_MSP_PATH_INDEX) return BT_ERR_BUDGET;
     return bt_msp_merge_peel(dg, levels_out);
 }

@@ -46,6 +46,11 @@ struct bt_buf {
 int bt_buf_reserve(bt_buf& b, size_t bytes);
 void bt_buf_free(bt_buf& b);
 
+// XCDs of an MI355X (8 x 32 CUs). Workgroups of a launch are dispatched
+// round-robin over them, so a grid that is a multiple of kXcds takes the
+// same number of CUs on each.
+constexpr int kXcds = 8;
+
 // Claim stamps past the R real ones that the peel's claim steps use for
 // lanes with nothing to claim (one per thread and unroll slot of a block).
 constexpr uint32_t kClaimDummiesPerBlock = 2048;
@@ -101,6 +106,10 @@ struct bt_ctx {
     int batch_lanes = 2;           // batches: lanes (contexts on CU shares) run side by side
     int peel_blocks = 0;           // CUs the persistent peel uses (0: all)
     int peel_slice_ms = 2000;      // time slice of a persistent peel launch (0: one launch)
+    uint64_t peel_max_level = 0;   // msp stops after this level (0: never); BT_PARTIAL
+    int peel_c2cap = 128;          // both-frontier middles summed from the on-chip list
+    uint64_t peel_window_edges = uint64_t(4) << 20;  // live edges from which the collect uses a window
+    int peel_big_middles = 1024;   // blooms beyond this are walked by a whole workgroup
     // device blocks given back by freed graphs and batches, reused by the
     // next ones (hipFree would wait for every stream on the device)
     std::mutex cache_mu;
@@ -115,6 +124,8 @@ int bt_ctx_alloc(bt_ctx* c, size_t bytes, void** p, size_t* got);
 // Give a block back to the context's cache once the work queued on its
 // stream no longer needs it (the caller synchronises the stream first).
 void bt_ctx_release(bt_ctx* c, void* p, size_t bytes);
+// Free every cached block of the context (after its stream has drained).
+int bt_ctx_trim_cache(bt_ctx* c);
 
 // Device-resident graph plus the per-graph scratch of the algorithms.
 struct bt_dgraph {

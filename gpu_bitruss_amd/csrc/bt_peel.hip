@@ -106,6 +106,7 @@ struct MspCtl {
     unsigned long long live_n[2];    // plain graphs: lengths of the two collect lists
     unsigned long long high_n[2];    // plain graphs: lengths of the two lists above the window
     unsigned long long hp, W;        // current list above the window, the window's top
+    unsigned long long partial;      // stopped after max_level with live edges left
 };
 static_assert(sizeof(MspCtl) <= 128 * sizeof(uint64_t), "control block overlays dg->ctl");
 
@@ -131,8 +132,6 @@ struct MspArgs {
     uint32_t* live;
     uint32_t* live_cnt;
     MspCtl* ctl;
-    uint64_t* trace;       // optional: per sub-round (claimed, slowest-wave ticks)
-    uint32_t trace_cap;
     // batches (the graph is a disjoint union; each graph peels its own levels)
     uint32_t ngraph;
     const uint32_t* egraph;  // m: graph of each edge
@@ -1169,7 +1168,6 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
             int lb = 0;
             while (lb < 23 && (pm >> (lb + 1))) lb++;
             ctl->hist[lb]++;
-            if (a.trace && X < a.trace_cap) a.trace[X] = (uint64_t(nn) << 32) | (pm & 0xFFFFFFFFull);
         }
         return nn;
     };
@@ -1279,7 +1277,10 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
                 }
             }
             // a prefix of the decomposition only: levels up to max_level
-            if (a.max_level && L > a.max_level) break;
+            if (a.max_level && L > a.max_level) {
+                if (B == 0 && tid == 0) ctl->partial = 1;
+                break;
+            }
             // ---- collect: frontier of level L from the shared live list
             {
                 uint64_t const tp0 = wall_clock64();
@@ -1508,10 +1509,10 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
 }
 
 __global__ void k_msp_init(MspCtl* ctl, uint32_t* arrive, uint32_t n_arrive, uint64_t* glev, uint32_t n_glev) {
-    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_arrive) arrive[i] = 0;
-    if (i < n_glev) glev[i] = kInf;
-    if (i == 0) {
+    uint32_t const first = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    for (uint32_t i = first; i < n_arrive; i += stride) arrive[i] = 0;
+    for (uint32_t i = first; i < n_glev; i += stride) glev[i] = kInf;
+    if (first == 0) {
         unsigned long long* w = reinterpret_cast<unsigned long long*>(ctl);
         for (size_t j = 0; j < sizeof(MspCtl) / sizeof(unsigned long long); j++) w[j] = 0;
     }
@@ -1554,10 +1555,9 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     int per_cu = 0;
     bool const batch = dg->egraph != nullptr && dg->ngraph > 0 && c->batch_levels == BT_BATCH_LEVELS_PER_GRAPH;
     // the window for plain graphs with many edges (the live edges at the
-    // start are at most m); BT_PEEL_WINDOW_EDGES lowers the threshold, to
-    // exercise the window on small graphs
-    uint64_t window_edges = kWindowEdges;
-    if (const char* e = getenv("BT_PEEL_WINDOW_EDGES")) window_edges = uint64_t(std::max(0L, atol(e)));
+    // start are at most m); BT_OPT_PEEL_WINDOW_EDGES moves the threshold
+    // (0 exercises the window on small graphs)
+    uint64_t const window_edges = c->peel_window_edges;
     bool const window = !batch && m > window_edges;
     const void* kern = batch    ? reinterpret_cast<const void*>(k_msp_persist<true>)
                        : window ? reinterpret_cast<const void*>(k_msp_persist<false, true>)
@@ -1569,10 +1569,9 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     uint32_t kmax = 0;
     BT_HIP_TRY(hipMemcpyAsync(&kmax, dg->ctl + kCtlBloomMax, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     BT_HIP_TRY(hipStreamSynchronize(s));
-    // BT_PEEL_C2CAP (0..128) lowers the LDS list's share, to exercise the
-    // sorted lists on small graphs
-    uint32_t c2cap = uint32_t(kC2Cap);
-    if (const char* e = getenv("BT_PEEL_C2CAP")) c2cap = uint32_t(std::min<long>(std::max(0L, atol(e)), kC2Cap));
+    // BT_OPT_PEEL_C2CAP (0..128) lowers the LDS list's share (0 exercises
+    // the sorted lists on small graphs)
+    uint32_t const c2cap = uint32_t(std::min(std::max(c->peel_c2cap, 0), kC2Cap));
     uint32_t const c2s_stride = kmax > c2cap ? (3 * kmax + 63) & ~63u : 0u;
     if (c2s_stride) BT_TRY(bt_buf_reserve(ix.c2s, sizeof(uint32_t) * size_t(G) * kWPB * c2s_stride));
     // claim stamps: R real ones, then the dummies (both sized by the index build)
@@ -1617,13 +1616,13 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     a.c2s_stride = c2s_stride;
     a.c2cap = c2cap;
     a.window_min = uint32_t(std::min<uint64_t>(window_edges / 4, 0xFFFFFFFFull));
-    // blooms beyond the register tiers (BT_PEEL_BIG_MIDDLES: another bound)
-    a.big_min = 16 * kWave;
-    if (const char* e = getenv("BT_PEEL_BIG_MIDDLES")) a.big_min = uint32_t(std::max(1L, atol(e)));
-    // BT_PEEL_MAX_LEVEL: stop after that level and report the state of the
-    // prefix (peeled edges their result, the others their current support),
-    // to check a prefix of a decomposition too long to run whole
-    if (const char* e = getenv("BT_PEEL_MAX_LEVEL")) a.max_level = batch ? 0 : uint64_t(std::max(0LL, atoll(e)));
+    // blooms beyond the register tiers (default 1,024 middles) are walked
+    // by the whole block
+    a.big_min = uint32_t(std::max(c->peel_big_middles, 1));
+    // BT_OPT_PEEL_MAX_LEVEL: stop after that level and report the state of
+    // the prefix (peeled edges their result, the others their current
+    // support) with BT_PARTIAL; plain graphs only (not batch unions)
+    a.max_level = dg->ngraph == 0 ? c->peel_max_level : 0;
     a.id_bits = 1;
     while (a.id_bits < 32 && ((m ? m - 1 : 0) >> a.id_bits)) a.id_bits++;
     if (batch) {
@@ -1633,15 +1632,8 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
         a.glev = static_cast<uint64_t*>(ix.glev.ptr);
         a.any = reinterpret_cast<uint32_t*>(seg + o_any);
     }
-    static bt_buf trace_buf;
-    bool const tracing = getenv("BT_PEEL_TRACE_FILE") != nullptr;
-    a.diag = (getenv("BT_PEEL_TRACE") || tracing) ? 1u : 0u;
-    if (tracing) {
-        a.trace_cap = m + 8;
-        BT_TRY(bt_buf_reserve(trace_buf, sizeof(uint64_t) * a.trace_cap));
-        a.trace = static_cast<uint64_t*>(trace_buf.ptr);
-        BT_HIP_TRY(hipMemsetAsync(a.trace, 0, sizeof(uint64_t) * a.trace_cap, s));
-    }
+    // per-phase timing diagnostics on stderr (results are the same)
+    a.diag = getenv("BT_PEEL_TRACE") ? 1u : 0u;
     k_msp_init<<<grid_for(std::max<size_t>(n_arrive, n_glev), 256, 1u << 16), 256, 0, s>>>(
         ctl, a.arrive, uint32_t(n_arrive), a.glev, n_glev);
     BT_HIP_TRY(hipGetLastError());
@@ -1711,7 +1703,9 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     MspCtl h{};
     BT_HIP_TRY(hipMemcpyAsync(&h, ctl, sizeof(MspCtl), hipMemcpyDeviceToHost, s));
     BT_HIP_TRY(hipStreamSynchronize(s));
-    if (h.err) return BT_ERR_INTERNAL;
+    // a barrier that timed out: on part of the device, other work held CUs
+    // the peel's workgroups needed (they were not resident together)
+    if (h.err) return G < unsigned(c->cu_count) ? BT_ERR_BUSY : BT_ERR_INTERNAL;
     if (a.diag) {
         std::fprintf(stderr,
                      "[bt_peel] index %.3f ms; phases: collect %.3f ms (block0 work %.3f), peel %.3f ms (block0 work "
@@ -1725,17 +1719,6 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
             if (h.hist[i]) std::fprintf(stderr, " %d:%llu", i, h.hist[i]);
         std::fprintf(stderr, "\n");
     }
-    if (tracing) {
-        std::vector<uint64_t> tr(std::min<uint64_t>(h.subrounds + 2, a.trace_cap));
-        BT_HIP_TRY(hipMemcpy(tr.data(), a.trace, sizeof(uint64_t) * tr.size(), hipMemcpyDeviceToHost));
-        FILE* f = std::fopen(getenv("BT_PEEL_TRACE_FILE"), "w");
-        if (f) {
-            for (size_t x = 0; x < tr.size(); x++)
-                std::fprintf(f, "%zu %llu %llu\n", x, (unsigned long long)(tr[x] >> 32),
-                             (unsigned long long)(tr[x] & 0xFFFFFFFFull));
-            std::fclose(f);
-        }
-    }
     c->last = count_t;
     c->last.peel_ms = tm.ms(5, 7);
     c->last.peel_kernel_ms = tm.ms(6, 7);
@@ -1746,5 +1729,5 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     c->last.levels_scanned = h.levels;
     if (h.level_max > 0xFFFFFFFFull) return BT_ERR_OVERFLOW;
     if (levels_out) *levels_out = uint32_t(h.level_max);
-    return BT_OK;
+    return h.partial ? BT_PARTIAL : BT_OK;
 }

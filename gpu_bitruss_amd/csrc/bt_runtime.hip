@@ -73,8 +73,27 @@ int bt_ctx_alloc(bt_ctx* c, size_t bytes, void** p, size_t* got) {
             return BT_OK;
         }
     }
-    BT_HIP_TRY(hipMalloc(p, bytes));
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        // the device may be full of this context's cached blocks: give them
+        // back and try once more
+        (void)hipGetLastError();
+        BT_TRY(bt_ctx_trim_cache(c));
+        BT_HIP_TRY(hipMalloc(p, bytes));
+    }
     *got = bytes;
+    return BT_OK;
+}
+
+int bt_ctx_trim_cache(bt_ctx* c) {
+    // cached blocks may still be read by work queued on the stream
+    BT_HIP_TRY(hipStreamSynchronize(c->stream));
+    std::multimap<size_t, void*> drop;
+    {
+        std::lock_guard<std::mutex> lk(c->cache_mu);
+        drop.swap(c->cache);
+        c->cache_bytes = 0;
+    }
+    for (auto const& kv : drop) BT_HIP_TRY(hipFree(kv.second));
     return BT_OK;
 }
 
@@ -281,6 +300,22 @@ int bt_ctx_set_option(bt_ctx* ctx, int option, int64_t value) {
             if (value < 0) return BT_ERR_INVALID;
             ctx->batch_wedges = uint64_t(value);
             return BT_OK;
+        case BT_OPT_PEEL_MAX_LEVEL:
+            if (value < 0) return BT_ERR_INVALID;
+            ctx->peel_max_level = uint64_t(value);
+            return BT_OK;
+        case BT_OPT_PEEL_C2CAP:
+            if (value < 0 || value > 128) return BT_ERR_INVALID;
+            ctx->peel_c2cap = int(value);
+            return BT_OK;
+        case BT_OPT_PEEL_WINDOW_EDGES:
+            if (value < 0) return BT_ERR_INVALID;
+            ctx->peel_window_edges = uint64_t(value);
+            return BT_OK;
+        case BT_OPT_PEEL_BIG_MIDDLES:
+            if (value < 1 || value > 0x7FFFFFFF) return BT_ERR_INVALID;
+            ctx->peel_big_middles = int(value);
+            return BT_OK;
         default:
             return BT_ERR_INVALID;
     }
@@ -399,10 +434,11 @@ int bt_hidx(bt_ctx* ctx, bt_dgraph* dg, uint64_t* bitruss_out, uint32_t* rounds)
 int bt_msp(bt_ctx* ctx, bt_dgraph* dg, uint64_t* bitruss_out, uint32_t* levels) {
     BT_TRY(check_graph(ctx, dg));
     uint32_t l = 0;
-    BT_TRY(bt_msp_impl(dg, &l));
+    int const st = bt_msp_impl(dg, &l);
+    if (st != BT_OK && st != BT_PARTIAL) return st;
     BT_TRY(copy_out(dg, dg->res, bitruss_out));
     if (levels) *levels = l;
-    return BT_OK;
+    return st;
 }
 
 }  // extern "C"

@@ -193,10 +193,13 @@ int bt_serve_start(const bt_serve_config* cfg, bt_server** out) {
     if (st == BT_OK) {
         // disjoint CU shares: a quarter for the light lane, three eighths
         // for the heavy lane, the rest for large requests
+        // (each a multiple of the 8 XCDs, so that shares adding up to the
+        // CU count fit every XCD together)
         int const cus = s->ctx_light->cu_count;
-        int const light = s->cfg.light_cus ? int(s->cfg.light_cus) : std::max(1, cus / 4);
-        int const heavy = s->cfg.heavy_cus ? int(s->cfg.heavy_cus) : std::max(1, 3 * cus / 8);
-        int const large = s->cfg.large_cus ? int(s->cfg.large_cus) : std::max(1, cus - light - heavy);
+        auto xcd = [](int x) { return std::max(kXcds, x / kXcds * kXcds); };
+        int const light = xcd(s->cfg.light_cus ? int(s->cfg.light_cus) : cus / 4);
+        int const heavy = xcd(s->cfg.heavy_cus ? int(s->cfg.heavy_cus) : 3 * cus / 8);
+        int const large = xcd(s->cfg.large_cus ? int(s->cfg.large_cus) : cus - light - heavy);
         if (light > cus || heavy > cus || large > cus) st = BT_ERR_INVALID;
         // a share of all CUs is a cooperative launch on the whole device
         auto share = [cus](int x) { return x >= cus ? 0 : x; };
@@ -301,9 +304,12 @@ int bt_serve_wait(bt_server* s, uint64_t ticket, uint64_t* out, bt_batch_result*
         std::unique_lock<std::mutex> lk(s->mu);
         auto it = s->all.find(ticket);
         if (it == s->all.end()) return BT_ERR_INVALID;
+        // the request itself is held, not the iterator: the wait releases
+        // the lock, and a submit meanwhile may rehash the map
         r = it->second;
         s->cv_done.wait(lk, [&] { return r->done; });
-        s->all.erase(it);
+        // another waiter on the same ticket may have released it already
+        if (s->all.erase(ticket) == 0) return BT_ERR_INVALID;
     }
     if (r->status != BT_OK) return r->status;
     if (out && !r->out.empty()) std::memcpy(out, r->out.data(), sizeof(uint64_t) * r->out.size());

@@ -2,10 +2,10 @@
 // semantics, .bin read/write, validation and the seeded generators.
 //
 // Reference behaviour followed (file:line in ShunyangLi/GPU_Bitruss):
-//   process_graph      src/graph/graph.cpp:128-296
-//   graph_to_bin       src/graph/graph.cpp:347-372
-//   load_graph_bin     src/graph/graph.cpp:389-429
-//   check_graph        src/graph/graph.cpp:67-106
+//   process_graph      src/graph/graph.cpp:74-242
+//   graph_to_bin       src/graph/graph.cpp:293-318
+//   load_graph_bin     src/graph/graph.cpp:335-375
+//   check_graph        src/graph/graph.cpp:13-52
 #include "bt_graph.h"
 
 #include <algorithm>
@@ -43,7 +43,7 @@ int alloc_graph(bt_host_graph* g, uint32_t n, uint32_t m) {
     return BT_OK;
 }
 
-// The lower-side edge-id lookup of process_graph (graph.cpp:259-278):
+// The lower-side edge-id lookup of process_graph (graph.cpp:196-228):
 // signed-int binary search with the `mid == 0` early exit.
 int64_t ref_binary_search(const uint32_t* nbr, uint32_t len, uint32_t value) {
     int left = 0;
@@ -227,6 +227,9 @@ const char* bt_status_str(int status) {
         case BT_ERR_OVERFLOW: return "count does not fit its type";
         case BT_ERR_INTERNAL: return "internal invariant failed";
         case BT_ERR_STATE: return "call not valid in the current state";
+        case BT_PARTIAL: return "partial result: msp stopped at the level bound (BT_OPT_PEEL_MAX_LEVEL)";
+        case BT_ERR_BUSY: return "persistent peel blocks were not co-resident (grid barrier timed out; CUs held by other work)";
+        case BT_ERR_SCRATCH: return "graph exceeds the static scratch of the allocation-free entry point";
         default: return "unknown status";
     }
 }
@@ -261,7 +264,7 @@ int bt_graph_from_edges(const uint32_t* raw_upper, const uint32_t* raw_lower, ui
         max_v = std::max(max_v, raw_lower[i]);
     }
     uint32_t const NONE = 0xFFFFFFFFu;
-    // first-appearance relabel (graph.cpp:146-168)
+    // first-appearance relabel (graph.cpp:92-114)
     std::vector<uint32_t> upper_ids(count ? size_t(max_u) + 1 : 0, NONE);
     std::vector<uint32_t> lower_ids(count ? size_t(max_v) + 1 : 0, NONE);
     std::vector<uint32_t> nu_of(count), nl_of(count);
@@ -284,7 +287,7 @@ int bt_graph_from_edges(const uint32_t* raw_upper, const uint32_t* raw_lower, ui
     out->m = m;
     out->support_max = 0;
 
-    // degrees and offsets (graph.cpp:211-219)
+    // degrees and offsets (graph.cpp:157-165)
     std::fill(out->degrees, out->degrees + n, 0u);
     for (uint64_t i = 0; i < count; i++) {
         out->degrees[nu_of[i]]++;
@@ -292,8 +295,8 @@ int bt_graph_from_edges(const uint32_t* raw_upper, const uint32_t* raw_lower, ui
     }
     out->offsets[0] = 0;
     for (uint32_t x = 0; x < n; x++) out->offsets[x + 1] = out->offsets[x] + out->degrees[x];
-    // fill lists (lower ids shifted by u_num, graph.cpp:172-185), then sort
-    // each list ascending (graph.cpp:188-197)
+    // fill lists (lower ids shifted by u_num, graph.cpp:118-131), then sort
+    // each list ascending (graph.cpp:134-143)
     std::vector<uint32_t> cursor(out->offsets, out->offsets + n);
     for (uint64_t i = 0; i < count; i++) {
         uint32_t const a = nu_of[i], b = u_num + nl_of[i];
@@ -304,7 +307,7 @@ int bt_graph_from_edges(const uint32_t* raw_upper, const uint32_t* raw_lower, ui
     for (int64_t x = 0; x < int64_t(n); x++)
         std::sort(out->neighbors + out->offsets[x], out->neighbors + out->offsets[x + 1]);
 
-    // edge ids in (upper id, neighbour) order (graph.cpp:232-248)
+    // edge ids in (upper id, neighbour) order (graph.cpp:178-194)
     uint32_t e = 0;
     for (uint32_t u = 0; u < u_num; u++) {
         for (uint32_t i = out->offsets[u]; i < out->offsets[u + 1]; i++) {
@@ -314,7 +317,7 @@ int bt_graph_from_edges(const uint32_t* raw_upper, const uint32_t* raw_lower, ui
             e++;
         }
     }
-    // lower-side ids by binary search in the upper list (graph.cpp:250-282)
+    // lower-side ids by binary search in the upper list (graph.cpp:196-228)
     int bad = 0;
 #pragma omp parallel for schedule(dynamic, 256) reduction(| : bad)
     for (int64_t vv = u_num; vv < int64_t(n); vv++) {
@@ -355,7 +358,7 @@ int bt_graph_read_text(const char* path, bt_host_graph* out) {
             if (!acc.empty() && acc.back() == '\n') break;
         }
         if (!got) break;
-        if (!acc.empty() && acc[0] == '%') continue;  // graph.cpp:147
+        if (!acc.empty() && acc[0] == '%') continue;  // graph.cpp:93
         const char* p = acc.c_str();
         while (*p == ' ' || *p == '\t' || *p == '\r' || *p == '\n') p++;
         if (*p == '\0') continue;  // blank line
@@ -392,10 +395,10 @@ int bt_graph_write_bin(const char* path, const bt_host_graph* g) {
     put(g->neighbors, 2 * m);
     put(g->offsets, n + 1);
     put(g->degrees, n);
-    put(zeros.data(), m);  // edge_support, always zero when written (graph.cpp:288)
+    put(zeros.data(), m);  // edge_support, always zero when written (graph.cpp:234, 312)
     put(g->edges, 2 * m);
     put(g->edge_ids, 2 * m);
-    put(zeros.data(), m);  // bitruss, always zero when written (graph.cpp:287)
+    put(zeros.data(), m);  // bitruss, always zero when written (graph.cpp:233, 315)
     ok = (std::fclose(f) == 0) && ok;
     return ok ? BT_OK : BT_ERR_IO;
 }

@@ -156,7 +156,7 @@ void device_info(int id) {
 }  // namespace
 
 int main(int argc, char** argv) {
-    std::setlocale(LC_ALL, "");  // main.cu:34-35 (digit grouping in %'d)
+    std::setlocale(LC_ALL, "");  // main.cu:33-35 (digit grouping in %'d)
     Args a;
     if (parse(argc, argv, a)) {
         LOG_ERROR("error: invalid arguments");
@@ -177,7 +177,7 @@ int main(int argc, char** argv) {
     if (!a.graph_used) return 0;
 
     bt_host_graph g{};
-    if (a.bin_used) {  // main.cu:83-90: convert and exit
+    if (a.bin_used) {  // main.cu:79-87: convert and exit
         LOG_INFO("processing graph file: %s", a.graph.c_str());
         int st = bt_graph_read_text(a.graph.c_str(), &g);
         if (st != BT_OK) {
@@ -194,7 +194,7 @@ int main(int argc, char** argv) {
         }
         return 0;
     }
-    if (a.graph.find(".bin") == std::string::npos) {  // graph.cpp:325-328
+    if (a.graph.find(".bin") == std::string::npos) {  // graph.cpp:271-274
         LOG_ERROR("graph file should be binary file");
         return EXIT_FAILURE;
     }
@@ -226,7 +226,7 @@ int main(int argc, char** argv) {
     if ((st = bt_graph_upload(ctx, &g, &dg)) != BT_OK) fail("bt_graph_upload", st);
     std::vector<uint64_t> out(g.m ? g.m : 1, 0);
     std::string algo = a.algo;
-    if (algo != "ebfc" && algo != "bfc" && algo != "hidx") algo = "msp";  // main.cu:327-330
+    if (algo != "ebfc" && algo != "bfc" && algo != "hidx") algo = "msp";  // main.cu:126-127
     uint64_t total = 0, smax = 0;
     uint32_t iters = 0;
     bool have_array = true;

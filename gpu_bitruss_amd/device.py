@@ -83,6 +83,23 @@ class Device:
     def set_index_mem_pct(self, pct: int) -> None:
         _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(self.handle, _lib.OPT_INDEX_MEM_PCT, pct))
 
+    def set_peel_max_level(self, level: int) -> None:
+        """msp stops after this level (0: never); msp() then reports the
+        prefix state with ``partial`` set on the device graph."""
+        _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(self.handle, _lib.OPT_PEEL_MAX_LEVEL, level))
+
+    def set_peel_c2cap(self, cap: int) -> None:
+        """Both-frontier middles summed on chip (0..128; 0 forces the sorted lists)."""
+        _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(self.handle, _lib.OPT_PEEL_C2CAP, cap))
+
+    def set_peel_window_edges(self, edges: int) -> None:
+        """Live edges from which the level collect uses a support window."""
+        _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(self.handle, _lib.OPT_PEEL_WINDOW_EDGES, edges))
+
+    def set_peel_big_middles(self, middles: int) -> None:
+        """Blooms with more middles are walked by a whole workgroup."""
+        _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(self.handle, _lib.OPT_PEEL_BIG_MIDDLES, middles))
+
     def upload(self, g: Graph) -> "DeviceGraph":
         return DeviceGraph(self, g)
 
@@ -153,9 +170,15 @@ class DeviceGraph:
         return (a[: self.m] if a is not None else None), r.value
 
     def msp(self, copy: bool = True):
+        """bt_msp. With a level bound on the device (set_peel_max_level) a
+        decomposition cut short returns its prefix state and sets
+        ``self.partial``."""
         a, p = self._out(copy)
         lv = C.c_uint32()
-        _lib.check("bt_msp", _lib.lib().bt_msp(self.dev.handle, self.handle, p, C.byref(lv)))
+        st = _lib.lib().bt_msp(self.dev.handle, self.handle, p, C.byref(lv))
+        self.partial = st == _lib.BT_PARTIAL
+        if not self.partial:
+            _lib.check("bt_msp", st)
         return (a[: self.m] if a is not None else None), lv.value
 
     def free(self) -> None:

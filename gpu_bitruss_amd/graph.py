@@ -161,7 +161,7 @@ class Graph:
         hg.edge_ids = _as_u32p(self.edge_ids)
         return hg
 
-    # -- I/O (graph.cpp:347-429) ------------------------------------------------
+    # -- I/O (graph.cpp:293-375) ------------------------------------------------
     def graph_to_bin(self, filename: str) -> None:
         hg = self.host_struct()
         _lib.check("bt_graph_write_bin", _lib.lib().bt_graph_write_bin(filename.encode(), C.byref(hg)))

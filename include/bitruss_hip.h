@@ -112,9 +112,25 @@ int bt_ctx_last_timings(const bt_ctx* ctx, bt_timings* out);
  *                          side by side. Results are the same. Streams beyond
  *                          the process's hardware queues (GPU_MAX_HW_QUEUES,
  *                          4 by default) share queues, which runs their
- *                          kernels one after the other. */
+ *                          kernels one after the other.
+ *   BT_OPT_PEEL_MAX_LEVEL  msp of a plain graph stops after this level (0, the
+ *                          default: never) and bt_msp returns BT_PARTIAL with
+ *                          the state of that prefix: the edges peeled so far
+ *                          hold their result, the others their current
+ *                          support. For checking a prefix of a decomposition
+ *                          too long to compare whole.
+ *   BT_OPT_PEEL_C2CAP      both-frontier middles of a bloom summed from the
+ *                          on-chip list before the sorted lists take over
+ *                          (0..128, default 128; 0 forces the sorted lists).
+ *                          Results are the same.
+ *   BT_OPT_PEEL_WINDOW_EDGES  live edges from which the level collect scans a
+ *                          support window (default 4,194,304; 0 uses it on
+ *                          every graph). Results are the same.
+ *   BT_OPT_PEEL_BIG_MIDDLES  blooms with more middles are walked by a whole
+ *                          workgroup (default 1,024). Results are the same. */
 enum { BT_OPT_MSP_PATH = 1, BT_OPT_INDEX_MEM_PCT = 2, BT_OPT_BATCH_WEDGES = 3, BT_OPT_BATCH_LEVELS = 4,
-       BT_OPT_PEEL_BLOCKS = 5, BT_OPT_PEEL_SLICE_MS = 6, BT_OPT_BATCH_LANES = 7 };
+       BT_OPT_PEEL_BLOCKS = 5, BT_OPT_PEEL_SLICE_MS = 6, BT_OPT_BATCH_LANES = 7, BT_OPT_PEEL_MAX_LEVEL = 8,
+       BT_OPT_PEEL_C2CAP = 9, BT_OPT_PEEL_WINDOW_EDGES = 10, BT_OPT_PEEL_BIG_MIDDLES = 11 };
 enum { BT_MSP_PATH_AUTO = 0, BT_MSP_PATH_MERGE = 1, BT_MSP_PATH_INDEX = 2 };
 enum { BT_BATCH_LEVELS_PER_GRAPH = 0, BT_BATCH_LEVELS_UNION = 1 };
 int bt_ctx_set_option(bt_ctx* ctx, int option, int64_t value);
@@ -146,7 +162,12 @@ int bt_hidx(bt_ctx* ctx, bt_dgraph* dg, uint64_t* bitruss_out, uint32_t* rounds)
  * Replaces bitruss_msp (bitruss_msp.cu:440-583), which writes
  * g->bitruss. bitruss_out (m entries) may be NULL; levels (may be NULL)
  * receives the value the reference logs as "levels" (the largest
- * bitruss number, 0 for a butterfly-free graph). */
+ * bitruss number, 0 for a butterfly-free graph). With BT_OPT_PEEL_MAX_LEVEL
+ * set and levels beyond it, the result is a prefix state and the status is
+ * BT_PARTIAL (the outputs are written). BT_ERR_BUSY: a peel given part of
+ * the device (BT_OPT_PEEL_BLOCKS) found its workgroups not resident
+ * together twice (CUs held by other work, e.g. another process), once on
+ * its share and once on the whole device. */
 int bt_msp(bt_ctx* ctx, bt_dgraph* dg, uint64_t* bitruss_out, uint32_t* levels);
 
 /* ---- batches: many independent graphs decomposed together --------------
@@ -201,7 +222,8 @@ int bt_batch_run(bt_ctx* ctx, bt_batch* b, int algo, uint64_t* const* outs, bt_b
  * requests with fewer than light_wedges priority wedges go to the light
  * lane, the others to the heavy lane; large requests are batched on a lane
  * of their own. The lanes' persistent peels run on disjoint CU shares
- * (light_cus, heavy_cus, large_cus), so no lane waits for another's. A request is
+ * (light_cus, heavy_cus, large_cus, each rounded down to a multiple of the
+ * 8 XCDs, at least 8), so no lane waits for another's. A request is
  * admitted when the estimated device bytes of the running requests plus its
  * own fit budget_bytes (the graph, ~140 B per edge, and its bloom index,
  * ~48 B per priority wedge); a request that alone exceeds the budget is

@@ -34,7 +34,16 @@ enum bt_status {
     BT_ERR_BUDGET = 6,      /* request exceeds the device-memory budget   */
     BT_ERR_OVERFLOW = 7,    /* a count does not fit its type              */
     BT_ERR_INTERNAL = 8,    /* an internal invariant failed               */
-    BT_ERR_STATE = 9        /* call not valid in the current state        */
+    BT_ERR_STATE = 9,       /* call not valid in the current state        */
+    BT_PARTIAL = 10,        /* msp stopped at the level bound of
+                               BT_OPT_PEEL_MAX_LEVEL: the results are the
+                               state of that prefix, not the decomposition */
+    BT_ERR_BUSY = 11,       /* the blocks of a persistent peel on part of
+                               the device were not resident together (its
+                               grid barrier timed out): other work held
+                               the CUs it was given                         */
+    BT_ERR_SCRATCH = 12     /* the graph needs more than the fixed static
+                               scratch of an allocation-free entry point    */
 };
 
 /* Text for a status code (static storage, never NULL). */
@@ -62,7 +71,7 @@ typedef struct bt_edge_list {
 } bt_edge_list;
 
 /* Build the CSR from raw pairs with the reference's process_graph semantics
- * (src/graph/graph.cpp:128-296): first-appearance relabel of upper and of
+ * (src/graph/graph.cpp:74-242): first-appearance relabel of upper and of
  * lower raw ids, lower ids shifted by u_num, sorted lists, edge ids in
  * (upper id, neighbour) order, lower-side ids by the same binary search.
  * Raw ids must be < BT_MAX_IDS. Duplicate pairs are kept, as in the
@@ -72,13 +81,13 @@ int bt_graph_from_edges(const uint32_t* raw_upper, const uint32_t* raw_lower,
 
 /* Parse a KONECT-style text edge list (lines starting with '%' skipped, the
  * first two integers of each line are "upper lower") and build the CSR as
- * bt_graph_from_edges does (src/graph/graph.cpp:128-170). Blank lines are
+ * bt_graph_from_edges does (src/graph/graph.cpp:74-114). Blank lines are
  * skipped; a non-blank line without two non-negative integers is
  * BT_ERR_INVALID. */
 int bt_graph_read_text(const char* path, bt_host_graph* out);
 
 /* Read / write the reference .bin layout byte for byte
- * (src/graph/graph.cpp:347-372 write, 389-429 read). The edge_support and
+ * (src/graph/graph.cpp:293-318 write, 335-375 read). The edge_support and
  * bitruss arrays of the file are written as zeros and skipped on read.
  * The reader validates sizes and CSR invariants and fails with
  * BT_ERR_INVALID instead of reading out of bounds. */
@@ -87,7 +96,7 @@ int bt_graph_write_bin(const char* path, const bt_host_graph* g);
 
 /* Structural validation: offsets monotone and consistent with degrees,
  * ids in range, sorted lists, and the edge-id consistency that the
- * reference's check_graph tests (src/graph/graph.cpp:67-106). */
+ * reference's check_graph tests (src/graph/graph.cpp:13-52). */
 int bt_graph_check(const bt_host_graph* g);
 
 /* Priority wedges of a graph: #{(u,v,w): v in N(u), w in N(v), v < u,

@@ -36,9 +36,18 @@ def test_every_declared_symbol_is_exported():
 
 def test_status_strings():
     L = _lib.lib()
-    for code in range(0, 10):
-        assert L.bt_status_str(code)
+    for code in range(0, 13):  # BT_OK .. BT_ERR_SCRATCH
+        assert L.bt_status_str(code) and L.bt_status_str(code) != b"unknown status", code
     assert L.bt_status_str(12345) == b"unknown status"
+    assert (_lib.BT_PARTIAL, _lib.BT_ERR_BUSY, _lib.BT_ERR_SCRATCH) == (10, 11, 12)
+
+
+def test_options_are_checked():
+    """Context options reject out-of-range values before any device work
+    (the context handle is checked first: no device is needed)."""
+    L = _lib.lib()
+    for opt in (_lib.OPT_PEEL_MAX_LEVEL, _lib.OPT_PEEL_C2CAP, _lib.OPT_PEEL_WINDOW_EDGES, _lib.OPT_PEEL_BIG_MIDDLES):
+        assert L.bt_ctx_set_option(None, opt, 1) == 1
 
 
 @pytest.mark.skipif(gb.device_count() > 0, reason="checks the no-device path")
@@ -71,7 +80,7 @@ def test_coh_cli(tmp_path):
     assert r.returncode == 0, r.stdout
     h = gb.Graph(str(b), False)
     assert h.m == 10
-    # non-.bin input is refused (graph.cpp:325-328)
+    # non-.bin input is refused (graph.cpp:271-274)
     r = run_coh("--gpu", "--graph", str(txt))
     assert r.returncode == 1 and "binary file" in r.stdout
     # the CPU algorithms are not served by the product

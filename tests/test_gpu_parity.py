@@ -218,12 +218,17 @@ def test_msp_time_slices(dev):
 
 
 @pytest.mark.gpu
-def test_sorted_rule_lists(dev, monkeypatch):
+def test_sorted_rule_lists(dev):
     """With the LDS list of both-frontier middles cut to nothing, every bloom
     that needs the attribution rule sorts its both-frontier middles into id
     lists and sums the rule by binary searches: results stay bit-exact
     against the oracle and the 2M-edge golden."""
-    monkeypatch.setenv("BT_PEEL_C2CAP", "0")
+    with gb.Device(dev.device) as d:
+        d.set_peel_c2cap(0)
+        _sorted_rule_lists(d)
+
+
+def _sorted_rule_lists(dev):
     sorted_visits = 0
     graphs = [(f"quirk{s}", G.quirk_graph(s)) for s in G.QUIRK_SEEDS[:4] + G.ORDER_SEEDS[:2]]
     graphs += [("mix2", gb.Graph.generate("mix_graph", 2)), ("hub", G.hub_graph(5, 6000, 20000, 400, 7000))]
@@ -247,12 +252,17 @@ def test_sorted_rule_lists(dev, monkeypatch):
 
 
 @pytest.mark.gpu
-def test_msp_collect_window(dev, monkeypatch):
+def test_msp_collect_window(dev):
     """The collect window (a list of the live edges with support <= W, the
     others above it, split again when the window empties), which the peel
     uses on graphs with millions of edges, forced onto smaller graphs: the
     oracle's results and the 2M-edge golden."""
-    monkeypatch.setenv("BT_PEEL_WINDOW_EDGES", "0")
+    with gb.Device(dev.device) as d:
+        d.set_peel_window_edges(0)
+        _msp_collect_window(d)
+
+
+def _msp_collect_window(dev):
     for name, g in [("quirk%d" % s, G.quirk_graph(s)) for s in G.QUIRK_SEEDS[:3]] + \
             [("mix2", gb.Graph.generate("mix_graph", 2)), ("hub", G.hub_graph(5, 6000, 20000, 400, 7000))]:
         say(f"window: {name} m={g.m}")
@@ -264,6 +274,7 @@ def test_msp_collect_window(dev, monkeypatch):
     gold = json.load(open(os.path.join(GOLDEN, "powerlaw_2m.json")))
     d = gb.Device(dev.device)
     try:
+        d.set_peel_window_edges(0)
         d.set_peel_slice_ms(5)  # resumes must keep the window's state
         dg = d.upload(gb.Graph.generate("powerlaw_2m"))
         say("window: powerlaw_2m in 5 ms slices")
@@ -275,23 +286,28 @@ def test_msp_collect_window(dev, monkeypatch):
 
 
 @pytest.mark.gpu
-def test_msp_level_prefix(dev, monkeypatch):
+def test_msp_level_prefix(dev):
     """A prefix of the decomposition (levels up to a bound; the peeled edges
     hold their result, the others their current support) equals the
     oracle's state after the same levels, with and without the collect
-    window."""
+    window; a cut-short call reports BT_PARTIAL, a bound beyond the last
+    level does not."""
     graphs = [("quirk%d" % s, G.quirk_graph(s)) for s in G.QUIRK_SEEDS[:3]]
     graphs += [("mix2", gb.Graph.generate("mix_graph", 2)), ("hub", G.hub_graph(5, 6000, 20000, 400, 7000))]
-    for window in ("0", "100000000"):
-        monkeypatch.setenv("BT_PEEL_WINDOW_EDGES", window)
-        for bound in (1, 3, 40):
-            monkeypatch.setenv("BT_PEEL_MAX_LEVEL", str(bound))
-            for name, g in graphs:
-                dg = dev.upload(g)
-                res, _ = dg.msp()
-                dg.free()
-                st, _, _ = oracle.msp_levels(g, 8, bound)
-                assert np.array_equal(res, st), (name, bound, window)
+    for window in (0, 100000000):
+        with gb.Device(dev.device) as d:
+            d.set_peel_window_edges(window)
+            for bound in (1, 3, 40):
+                d.set_peel_max_level(bound)
+                for name, g in graphs:
+                    dg = d.upload(g)
+                    res, _ = dg.msp()
+                    partial = dg.partial
+                    dg.free()
+                    st, _, _ = oracle.msp_levels(g, 8, bound)
+                    _, olevels, _ = oracle.msp(g, 8)
+                    assert np.array_equal(res, st), (name, bound, window)
+                    assert partial == (olevels > bound), (name, bound, window)
         say(f"level prefixes equal the oracle's (window threshold {window})")
 
 
@@ -356,17 +372,19 @@ def test_skewed_60m_golden(dev):
 
 
 @pytest.mark.gpu
-def test_skewed_60m_msp_prefix(dev, monkeypatch):
+def test_skewed_60m_msp_prefix(dev):
     """The 60M-edge graph's msp decomposition through its first levels
     (the whole one runs for hours): the state after them — the peeled edges'
     results and the others' current supports — against the oracle's."""
     gold = json.load(open(os.path.join(GOLDEN, "skewed_60m.json")))
     if "msp_prefix_sha256" not in gold:
         pytest.skip("the oracle's prefix state of skewed_60m is not recorded (make_golden.py --msp-prefix)")
-    monkeypatch.setenv("BT_PEEL_MAX_LEVEL", str(gold["msp_prefix_levels"]))
     g = gb.Graph.generate("skewed_60m")
-    dg = dev.upload(g)
-    say(f"skewed_60m msp through level {gold['msp_prefix_levels']}")
-    res, _ = dg.msp()
-    dg.free()
+    with gb.Device(dev.device) as d:
+        d.set_peel_max_level(gold["msp_prefix_levels"])
+        dg = d.upload(g)
+        say(f"skewed_60m msp through level {gold['msp_prefix_levels']}")
+        res, _ = dg.msp()
+        assert dg.partial
+        dg.free()
     assert digest(res) == gold["msp_prefix_sha256"]

@@ -13,7 +13,7 @@ from tests import graphs as G
 
 
 def py_process_graph(pairs):
-    """Independent restatement of process_graph (graph.cpp:128-296):
+    """Independent restatement of process_graph (graph.cpp:74-242):
     first-appearance ids, lower shifted by u_num, sorted lists, edge ids in
     (upper, neighbour) order."""
     up, lo = {}, {}
@@ -107,7 +107,7 @@ def test_bin_reader_rejects_bad_files(tmp_path):
     with pytest.raises(gb.BitrussError):
         gb.Graph(str(tmp_path / "missing.bin"), False)
     with pytest.raises(ValueError):
-        gb.Graph(str(tmp_path / "x.txt"), False)  # graph.cpp:325-328
+        gb.Graph(str(tmp_path / "x.txt"), False)  # graph.cpp:271-274
 
 
 def test_text_parsing(tmp_path):
