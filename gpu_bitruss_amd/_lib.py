@@ -163,6 +163,7 @@ def lib() -> C.CDLL:
         "bt_ctx_destroy": (C.c_int, [vp]),
         "bt_ctx_last_timings": (C.c_int, [vp, C.POINTER(BtTimings)]),
         "bt_ctx_set_option": (C.c_int, [vp, C.c_int, C.c_int64]),
+        "bt_ctx_trim": (C.c_int, [vp]),
         "bt_graph_upload": (C.c_int, [vp, C.POINTER(BtHostGraph), C.POINTER(vp)]),
         "bt_graph_free": (C.c_int, [vp]),
         "bt_bfc": (C.c_int, [vp, vp, u64p]),

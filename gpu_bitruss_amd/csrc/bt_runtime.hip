@@ -321,6 +321,20 @@ int bt_ctx_set_option(bt_ctx* ctx, int option, int64_t value) {
     }
 }
 
+int bt_ctx_trim(bt_ctx* ctx) {
+    BT_TRY(check_ctx(ctx));
+    BT_HIP_TRY(hipSetDevice(ctx->device));
+    BT_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    bt_buf* bufs[] = {&ctx->sort_tmp, &ctx->pool, &ctx->ix.keys0, &ctx->ix.keys1, &ctx->ix.vals0, &ctx->ix.vals1,
+                      &ctx->ix.bid, &ctx->ix.bstart, &ctx->ix.bk, &ctx->ix.bclaim, &ctx->ix.inc_off,
+                      &ctx->ix.inc_cur, &ctx->ix.inc, &ctx->ix.touched, &ctx->ix.area, &ctx->ix.bgraph,
+                      &ctx->ix.glev, &ctx->ix.live, &ctx->ix.seg, &ctx->ix.c2s, &ctx->ix.hdirty};
+    for (bt_buf* b : bufs) bt_buf_free(*b);
+    ctx->ix.R = 0;
+    ctx->ix.NB = 0;
+    return bt_ctx_trim_cache(ctx);
+}
+
 int bt_ctx_last_timings(const bt_ctx* ctx, bt_timings* out) {
     if (!ctx || !out) return BT_ERR_INVALID;
     *out = ctx->last;

@@ -100,6 +100,10 @@ class Device:
         """Blooms with more middles are walked by a whole workgroup."""
         _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(self.handle, _lib.OPT_PEEL_BIG_MIDDLES, middles))
 
+    def trim(self) -> None:
+        """Give back the device memory the context caches between calls."""
+        _lib.check("bt_ctx_trim", _lib.lib().bt_ctx_trim(self.handle))
+
     def upload(self, g: Graph) -> "DeviceGraph":
         return DeviceGraph(self, g)
 

@@ -76,6 +76,11 @@ int bt_device_info(int device, bt_device_props* out);
 int bt_ctx_create(int device, bt_ctx** out);
 int bt_ctx_destroy(bt_ctx* ctx);
 int bt_ctx_last_timings(const bt_ctx* ctx, bt_timings* out);
+/* Give back the device memory the context keeps between calls: the blocks
+ * of freed graphs and batches it caches for reuse, and its index and
+ * scratch buffers (they grow again on the next call). No counterpart in the
+ * reference, which frees everything per call. */
+int bt_ctx_trim(bt_ctx* ctx);
 
 /* Context options (no counterpart in the reference, whose launch shapes are
  * compile-time macros in src/util/config.h:6-13).
