@@ -107,6 +107,10 @@ struct MspCtl {
     unsigned long long high_n[2];    // plain graphs: lengths of the two lists above the window
     unsigned long long hp, W;        // current list above the window, the window's top
     unsigned long long partial;      // stopped after max_level with live edges left
+    // diagnostics: per peel phase the slowest block's claim step and whole
+    // work (max over blocks, summed by block 0), middles walked by waves
+    // and by whole blocks, large-bloom visits
+    unsigned long long cmax[2], cmax_sum, wmax[2], wmax_sum, mid_wave, mid_block, n_block;
 };
 static_assert(sizeof(MspCtl) <= 128 * sizeof(uint64_t), "control block overlays dg->ctl");
 
@@ -953,6 +957,17 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
     uint64_t W = a.resume ? ctl->W : 0ull;  // top of the collect window (plain graphs)
     uint32_t nt = 0;            // claimed blooms of sub-round X
     uint64_t t_col = 0, t_pe = 0, w_col = 0, w_pe = 0, pmax_sum = 0;
+    uint64_t d_wave = 0, d_block = 0, d_nblock = 0;  // diagnostics: middles walked (lane 0s count)
+    // adds this thread's middle counts to the control block (diagnostics)
+    auto flush_diag = [&]() {
+        if (!a.diag) return;
+        if ((tid & (kWave - 1)) == 0 && d_wave) afetch_add(&ctl->mid_wave, (unsigned long long)d_wave);
+        if (tid == 0 && d_block) {
+            afetch_add(&ctl->mid_block, (unsigned long long)d_block);
+            afetch_add(&ctl->n_block, (unsigned long long)d_nblock);
+        }
+        d_wave = d_block = d_nblock = 0;
+    };
 
     // ---- one peel sub-round X over the claimed blooms; level_of(bg) is the
     // level of the bloom's graph (bg = its graph in a batch, else 0); the
@@ -1027,6 +1042,7 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
                 }
             }
             dec.L = level_of(bg);
+            if (a.diag) d_wave += kb;
             // a bloom frees at most two edges per middle
             xb.ensure(2 * (kb < uint32_t(16 * kWave) ? kb : uint32_t(16 * kWave)), X + 1, a, out);
             bool done = false;
@@ -1035,43 +1051,62 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
             else if (kb <= uint32_t(16 * kWave)) done = process_bloom<16>(a, s, kb, up, r0, X, dec, c2, xb, out);
             if (!done) process_large(a, s, kb, up, X, dec, c2, xb, out);
         }
-        // ---- the large blooms, each by the whole block: the waves count
-        // the middle groups of interleaved 64-middle chunks, then apply the
-        // decrements; the both-frontier middles for the rule are gathered in
-        // LDS. A bloom whose both-frontier middles overflow that list goes
-        // to one wave (process_large: sorted lists). The bloom is compacted
-        // (by one wave) once at most half of it is left, so the walks stay
-        // proportional to its live middles.
+        // ---- the large blooms, each by the whole block: a counting pass
+        // over the middles (UB per thread in flight, so a block step covers
+        // UB * 512 middles with one round of dependent loads), then a
+        // decrement pass in the same layout; the both-frontier middles for
+        // the rule are gathered in LDS. A bloom whose both-frontier middles
+        // overflow that list goes to one wave (process_large: sorted lists).
+        // Once at most half of it is left the decrement pass also compacts
+        // it (each block step reads its middles before any is written, and
+        // writes land below them), so the walks stay proportional to its
+        // live middles; middles keep no order.
         __syncthreads();
         uint32_t const nbig = s_nbig < kBigCap ? s_nbig : kBigCap;
+        constexpr int UB = 4;
         for (uint32_t bi = 0; bi < nbig; bi++) {
             uint32_t const s = s_big[bi];
             uint32_t const hdr = ald(&a.hdr[s]);
             uint32_t const kb = hdr & 0x7FFFFFFFu, up = hdr >> 31;
             uint64_t* rec = a.rec + s;
             dec.L = level_of(s_big_bg[bi]);
+            if (a.diag) {
+                d_block += kb;
+                d_nblock++;
+            }
             if (tid < 4) s_bcnt[tid] = 0;
             __syncthreads();
             uint32_t w0 = 0, w1 = 0, wp = 0;
-            for (uint32_t j0 = uint32_t(wv) * kWave; j0 < kb; j0 += kMB) {
-                uint32_t const j = j0 + lane;
-                int c = -1;
-                uint64_t r = 0;
-                if (j < kb) {
-                    r = ald(&rec[j]);
-                    c = classify(ald(&a.stamp[uint32_t(r >> 32)]), ald(&a.stamp[uint32_t(r)]), X);
+            for (uint32_t j0 = uint32_t(wv) * kWave; j0 < kb; j0 += UB * kMB) {
+                uint64_t r[UB];
+                uint32_t s1[UB], s2[UB];
+#pragma unroll
+                for (int u = 0; u < UB; u++) {
+                    uint32_t const j = j0 + u * kMB + lane;
+                    r[u] = j < kb ? ald(&rec[j]) : 0;
                 }
-                uint64_t const b2 = __ballot(c == 2);
-                if (b2) {
-                    uint32_t base = 0;
-                    if (lane == 0) base = atomicAdd(&s_bcnt[2], uint32_t(__popcll(b2)));
-                    base = __builtin_amdgcn_readfirstlane(base);
-                    uint32_t const pos = base + __popcll(b2 & lanemask_lt());
-                    if (c == 2 && pos < uint32_t(kC2Cap)) s_bc2[pos] = r;
+#pragma unroll
+                for (int u = 0; u < UB; u++) {
+                    uint32_t const j = j0 + u * kMB + lane;
+                    s1[u] = j < kb ? ald(&a.stamp[uint32_t(r[u] >> 32)]) : 0u;
+                    s2[u] = j < kb ? ald(&a.stamp[uint32_t(r[u])]) : 0u;
                 }
-                w0 += __popcll(__ballot(c == 0));
-                w1 += __popcll(__ballot(c == 1));
-                wp += __popcll(__ballot(c == 2 && uint32_t(r >> 32) < uint32_t(r)));
+#pragma unroll
+                for (int u = 0; u < UB; u++) {
+                    uint32_t const j = j0 + u * kMB + lane;
+                    int const c = j < kb ? classify(s1[u], s2[u], X) : -1;
+                    uint64_t const b2 = __ballot(c == 2);
+                    if (b2) {
+                        uint32_t base = 0;
+                        if (lane == 0) base = atomicAdd(&s_bcnt[2], uint32_t(__popcll(b2)));
+                        base = __builtin_amdgcn_readfirstlane(base);
+                        uint32_t const pos = base + __popcll(b2 & lanemask_lt());
+                        if (c == 2 && pos < uint32_t(kC2Cap)) s_bc2[pos] = r[u];
+                    }
+                    w0 += __popcll(__ballot(c == 0));
+                    w1 += __popcll(__ballot(c == 1));
+                    wp += __popcll(__ballot(c == 2 && uint32_t(r[u] >> 32) < uint32_t(r[u])));
+                }
             }
             if (lane == 0) {
                 atomicAdd(&s_bcnt[0], w0);
@@ -1086,54 +1121,82 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
                 continue;
             }
             uint64_t const dec0 = uint64_t(n1) + n2;
-            for (uint32_t j0 = uint32_t(wv) * kWave; j0 < kb; j0 += kMB) {
-                xb.ensure(2 * kWave, X + 1, a, out);
-                uint32_t const j = j0 + lane;
-                int c = -1;
-                uint64_t r = 0;
-                uint32_t e1 = 0, e2 = 0, s1 = 0;
-                if (j < kb) {
-                    r = ald(&rec[j]);
-                    e1 = uint32_t(r >> 32);
-                    e2 = uint32_t(r);
-                    s1 = ald(&a.stamp[e1]);
-                    c = classify(s1, ald(&a.stamp[e2]), X);
+            // compact in this pass: a bloom at most half alive that keeps a
+            // butterfly (below two middles the top claim stamp retires it)
+            bool const compact = n0 >= 2 && 2 * n0 <= kb;
+            uint32_t wpos = 0;  // kept middles written so far (whole block)
+            for (uint32_t it0 = 0; it0 < kb; it0 += UB * kMB) {
+                xb.ensure(2 * UB * kWave, X + 1, a, out);
+                uint32_t const j0 = it0 + uint32_t(wv) * kWave;
+                uint64_t r[UB];
+                uint32_t s1[UB], s2[UB];
+                int c[UB];
+#pragma unroll
+                for (int u = 0; u < UB; u++) {
+                    uint32_t const j = j0 + u * kMB + lane;
+                    r[u] = j < kb ? ald(&rec[j]) : 0;
                 }
-                bool x1 = false, x2 = false;
-                uint32_t f = e1;
-                if (c == 0 && dec0) {
-                    x1 = dec(e1, dec0);
-                    x2 = dec(e2, dec0);
-                } else if (c == 1) {
-                    bool const froot = s1 != X;
-                    f = froot ? e1 : e2;
-                    uint32_t const g = froot ? e2 : e1;
-                    uint64_t const d = uint64_t(n0) + (n1 - 1) + rule_sum(s_bc2, n2, froot, up, g);
-                    if (d) x1 = dec(f, d);
+#pragma unroll
+                for (int u = 0; u < UB; u++) {
+                    uint32_t const j = j0 + u * kMB + lane;
+                    s1[u] = j < kb ? ald(&a.stamp[uint32_t(r[u] >> 32)]) : 0u;
+                    s2[u] = j < kb ? ald(&a.stamp[uint32_t(r[u])]) : 0u;
                 }
-                xb.put(x1, f);
-                xb.put(x2, e2);
+                // targets: the first is r-v (c = 0) or the survivor (c = 1),
+                // the second v-w (c = 0 only)
+                uint32_t t1[UB];
+                uint64_t d1[UB];
+                uint32_t nk = 0;
+#pragma unroll
+                for (int u = 0; u < UB; u++) {
+                    uint32_t const j = j0 + u * kMB + lane;
+                    c[u] = j < kb ? classify(s1[u], s2[u], X) : -1;
+                    uint32_t const e1 = uint32_t(r[u] >> 32), e2 = uint32_t(r[u]);
+                    t1[u] = e1;
+                    d1[u] = 0;
+                    if (c[u] == 0) {
+                        d1[u] = dec0;
+                        nk++;
+                    } else if (c[u] == 1) {
+                        bool const froot = s1[u] != X;  // the survivor is r-v
+                        uint32_t const g = froot ? e2 : e1;
+                        t1[u] = froot ? e1 : e2;
+                        d1[u] = uint64_t(n0) + (n1 - 1) + rule_sum(s_bc2, n2, froot, up, g);
+                    }
+                }
+                uint64_t o1[UB], o2[UB];
+#pragma unroll
+                for (int u = 0; u < UB; u++) {
+                    o1[u] = d1[u] ? dec.issue(t1[u], d1[u]) : 0;
+                    o2[u] = (c[u] == 0 && dec0) ? dec.issue(uint32_t(r[u]), dec0) : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < UB; u++) {
+                    bool const x1 = d1[u] && dec.settle(t1[u], d1[u], o1[u]);
+                    bool const x2 = c[u] == 0 && dec0 && dec.settle(uint32_t(r[u]), dec0, o2[u]);
+                    xb.put(x1, t1[u]);
+                    xb.put(x2, uint32_t(r[u]));
+                }
+                if (compact) {
+                    // every middle of this block step has been read (the
+                    // scan's barriers) before the kept ones are written
+                    uint32_t tot = 0;
+                    uint32_t pos = wpos + block_excl_scan<kMB>(nk, s_part, tot);
+#pragma unroll
+                    for (int u = 0; u < UB; u++)
+                        if (c[u] == 0) ast(&rec[pos++], r[u]);
+                    wpos += tot;
+                }
             }
             __syncthreads();
-            if (n0 < 2) {
-                // no butterfly is left: the top claim stamp keeps the bloom
-                // out of every later claim list
-                if (tid == 0) afetch_max(&a.claim[s], ~0u);
-            } else if (2 * n0 <= kb && wv == 0) {
-                uint32_t wpos = 0;
-                for (uint32_t j0 = 0; j0 < kb; j0 += kWave) {
-                    uint32_t const j = j0 + lane;
-                    int c = -1;
-                    uint64_t r = 0;
-                    if (j < kb) {
-                        r = ald(&rec[j]);
-                        c = classify(ald(&a.stamp[uint32_t(r >> 32)]), ald(&a.stamp[uint32_t(r)]), X);
-                    }
-                    uint64_t const keepm = __ballot(c == 0);
-                    if (c == 0) ast(&rec[wpos + __popcll(keepm & lanemask_lt())], r);
-                    wpos += __popcll(keepm);
+            if (tid == 0) {
+                if (n0 < 2) {
+                    // no butterfly is left: the top claim stamp keeps the
+                    // bloom out of every later claim list
+                    afetch_max(&a.claim[s], ~0u);
+                } else if (compact) {
+                    ast(&a.hdr[s], wpos | (up << 31));
                 }
-                if (lane == 0) ast(&a.hdr[s], wpos | (up << 31));
             }
             __syncthreads();
         }
@@ -1148,13 +1211,18 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
         } else {
             __syncthreads();
         }
+        uint64_t const tc0 = wall_clock64();
         claim_block(a, X + 1, out, s_xl, s_xn, s_incl, s_off, s_part);
         close_out(X + 1);
+        uint64_t const tp1 = wall_clock64();
         if (a.diag) {
             uint64_t const mx = ~block_min_all(~tw, s_red);  // slowest wave of the block
-            if (tid == 0) afetch_max(&ctl->pmax[X & 1], (unsigned long long)mx);
+            if (tid == 0) {
+                afetch_max(&ctl->pmax[X & 1], (unsigned long long)mx);
+                afetch_max(&ctl->cmax[X & 1], (unsigned long long)(tp1 - tc0));
+                afetch_max(&ctl->wmax[X & 1], (unsigned long long)(tp1 - tp0));
+            }
         }
-        uint64_t const tp1 = wall_clock64();
         if (a.slice_ticks && B == 0 && tid == 0) ast(&ctl->yield, tp1 - ctl->t_launch >= a.slice_ticks ? 1u : 0u);
         if (!grid_sync(a, G, ++epoch, &s_ok)) return ~0u;
         uint32_t const nn = open_list(X + 1);
@@ -1165,6 +1233,10 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
             unsigned long long const pm = ald(&ctl->pmax[X & 1]);
             ast(&ctl->pmax[X & 1], 0ull);
             pmax_sum += pm;
+            ctl->cmax_sum += ald(&ctl->cmax[X & 1]);
+            ast(&ctl->cmax[X & 1], 0ull);
+            ctl->wmax_sum += ald(&ctl->wmax[X & 1]);
+            ast(&ctl->wmax[X & 1], 0ull);
             int lb = 0;
             while (lb < 23 && (pm >> (lb + 1))) lb++;
             ctl->hist[lb]++;
@@ -1190,6 +1262,7 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
                     ctl->pmax_sum += pmax_sum;
                     ast(&ctl->yielded, 1u);
                 }
+                flush_diag();
                 return;
             }
             uint32_t const p = k & 1;
@@ -1497,6 +1570,7 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
             k++;
         }
     }
+    flush_diag();
     if (B == 0 && tid == 0) {
         ctl->subrounds = X - 1;
         ctl->levels = k;
@@ -1646,6 +1720,7 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     // events 6..7 bracket the persistent kernel's launches
     BT_TRY(tm.start(6));
     void* params[] = {&a};
+    bool plain_full = false;  // a whole-device peel launched without the cooperative guarantee
     {
         bt_cu_lease lease(c->device, c->cu_count, int(G));
         for (;; slices++) {
@@ -1657,8 +1732,16 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
                 // the CU count, and other kernels on the device finish by
                 // themselves.
                 BT_HIP_TRY(hipLaunchKernel(kern, dim3(G), dim3(kMB), params, 0, s));
-            } else {
-                BT_HIP_TRY(hipLaunchCooperativeKernel(kern, dim3(G), dim3(kMB), params, 0, s));
+            } else if (hipLaunchCooperativeKernel(kern, dim3(G), dim3(kMB), params, 0, s) != hipSuccess) {
+                // the runtime refuses cooperative launches while another
+                // process holds the device ("cannot be performed in the
+                // present state"): a plain launch of one block per CU, whose
+                // blocks are resident together unless that other process
+                // keeps CUs busy, in which case the bounded grid barrier
+                // ends the peel with BT_ERR_BUSY
+                (void)hipGetLastError();
+                BT_HIP_TRY(hipLaunchKernel(kern, dim3(G), dim3(kMB), params, 0, s));
+                plain_full = true;
             }
             if (!a.slice_ticks) break;
             uint64_t st[2 + 4];  // yield, yielded, r_X, r_k, r_L, r_epoch
@@ -1703,9 +1786,9 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     MspCtl h{};
     BT_HIP_TRY(hipMemcpyAsync(&h, ctl, sizeof(MspCtl), hipMemcpyDeviceToHost, s));
     BT_HIP_TRY(hipStreamSynchronize(s));
-    // a barrier that timed out: on part of the device, other work held CUs
-    // the peel's workgroups needed (they were not resident together)
-    if (h.err) return G < unsigned(c->cu_count) ? BT_ERR_BUSY : BT_ERR_INTERNAL;
+    // a barrier that timed out: in a plain launch, other work held CUs the
+    // peel's workgroups needed (they were not resident together)
+    if (h.err) return (G < unsigned(c->cu_count) || plain_full) ? BT_ERR_BUSY : BT_ERR_INTERNAL;
     if (a.diag) {
         std::fprintf(stderr,
                      "[bt_peel] index %.3f ms; phases: collect %.3f ms (block0 work %.3f), peel %.3f ms (block0 work "
@@ -1714,6 +1797,10 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
                      tm.ms(5, 6), h.t_collect * 1e-5, h.t_work_collect * 1e-5, h.t_peel * 1e-5, h.t_work_peel * 1e-5,
                      h.pmax_sum * 1e-5, h.subrounds, h.levels, G, (unsigned long long)ix.R,
                      (unsigned long long)ix.NB);
+        std::fprintf(stderr,
+                     "[bt_peel] per peel phase, slowest block: claim step %.3f ms, whole work %.3f ms (sums); "
+                     "middles walked by waves %llu, by whole blocks %llu in %llu visits\n",
+                     h.cmax_sum * 1e-5, h.wmax_sum * 1e-5, h.mid_wave, h.mid_block, h.n_block);
         std::fprintf(stderr, "[bt_peel] slowest-wave histogram (log2 10ns ticks: phases):");
         for (int i = 0; i < 24; i++)
             if (h.hist[i]) std::fprintf(stderr, " %d:%llu", i, h.hist[i]);
