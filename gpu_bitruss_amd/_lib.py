@@ -168,6 +168,7 @@ def lib() -> C.CDLL:
         "bt_graph_free": (C.c_int, [vp]),
         "bt_bfc": (C.c_int, [vp, vp, u64p]),
         "bt_ebfc": (C.c_int, [vp, vp, u64p, u64p, u64p]),
+        "bt_ebfc_static": (C.c_int, [vp, vp, u64p, u64p, u64p]),
         "bt_hidx": (C.c_int, [vp, vp, u64p, C.POINTER(C.c_uint32)]),
         "bt_msp": (C.c_int, [vp, vp, u64p, C.POINTER(C.c_uint32)]),
         "bt_batch_upload": (C.c_int, [vp, C.POINTER(BtHostGraph), C.c_size_t, C.POINTER(vp)]),

@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <mutex>
 #include <vector>
 
 #include "bt_common.h"
@@ -345,7 +346,7 @@ int bits_for(uint64_t x) {
 
 }  // namespace
 
-int bt_prep_priority(bt_dgraph* dg) {
+int bt_prep_priority(bt_dgraph* dg, const bt_fixed_scratch* fixed) {
     bt_ctx* c = dg->ctx;
     hipStream_t s = c->stream;
     uint32_t const n = dg->n, m = dg->m;
@@ -359,18 +360,25 @@ int bt_prep_priority(bt_dgraph* dg) {
         BT_HIP_TRY(rocprim::radix_sort_pairs(nullptr, t2, dg->r_key, dg->r_key2, dg->r_val2, dg->r_eid,
                                              2 * size_t(m), 0, 32 + nb, s));
     BT_HIP_TRY(rocprim::exclusive_scan(nullptr, t3, dg->list0, dg->r_off, 0u, n, rocprim::plus<uint32_t>(), s));
-    BT_TRY(bt_buf_reserve(c->sort_tmp, std::max(t1, std::max(t2, t3))));
+    size_t const t_need = std::max(t1, std::max(t2, t3));
+    void* tmp = nullptr;
+    if (fixed) {
+        if (t_need > fixed->bytes) return BT_ERR_SCRATCH;
+        tmp = fixed->ptr;
+    } else {
+        BT_TRY(bt_buf_reserve(c->sort_tmp, t_need));
+        tmp = c->sort_tmp.ptr;
+    }
     // rank = position in (degree, id) order
     k_rank_keys<<<grid_for(n, T, 1u << 30), T, 0, s>>>(n, dg->deg, dg->r_key);
-    BT_HIP_TRY(rocprim::radix_sort_keys(c->sort_tmp.ptr, t1, dg->r_key, dg->r_key2, n, 0, 64, s));
+    BT_HIP_TRY(rocprim::radix_sort_keys(tmp, t1, dg->r_key, dg->r_key2, n, 0, 64, s));
     k_rank_scatter<<<grid_for(n, T, 1u << 30), T, 0, s>>>(n, dg->u_num, dg->r_key2, dg->rank, dg->list0, dg->r_up);
-    BT_HIP_TRY(rocprim::exclusive_scan(c->sort_tmp.ptr, t3, dg->list0, dg->r_off, 0u, n,
-                                       rocprim::plus<uint32_t>(), s));
+    BT_HIP_TRY(rocprim::exclusive_scan(tmp, t3, dg->list0, dg->r_off, 0u, n, rocprim::plus<uint32_t>(), s));
     k_set_last<<<1, 64, 0, s>>>(dg->r_off, n, 2 * m);
     if (m) {
         // rows of the relabelled CSR: sort directed pairs by (rank, rank)
         k_pair_keys<<<grid_for(m, T, 1u << 30), T, 0, s>>>(m, dg->edges, dg->rank, dg->r_key, dg->r_val2);
-        BT_HIP_TRY(rocprim::radix_sort_pairs(c->sort_tmp.ptr, t2, dg->r_key, dg->r_key2, dg->r_val2, dg->r_eid,
+        BT_HIP_TRY(rocprim::radix_sort_pairs(tmp, t2, dg->r_key, dg->r_key2, dg->r_val2, dg->r_eid,
                                              2 * size_t(m), 0, 32 + nb, s));
         k_pair_unpack<<<grid_for(2ull * m, T, 1u << 30), T, 0, s>>>(2 * m, dg->r_key2, dg->r_nbr);
     }
@@ -380,7 +388,8 @@ int bt_prep_priority(bt_dgraph* dg) {
     return BT_OK;
 }
 
-int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total_out, uint64_t* smax_out, bool emit) {
+int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total_out, uint64_t* smax_out, bool emit,
+                  const bt_fixed_scratch* fixed) {
     bt_ctx* c = dg->ctx;
     hipStream_t s = c->stream;
     uint32_t const n = dg->n, m = dg->m;
@@ -399,7 +408,8 @@ int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total_out, uint64_t* s
         c->last.total_ms = tm.ms(0, 1);
         return BT_OK;
     }
-    BT_TRY(bt_prep_priority(dg));
+    if (fixed) emit = false;  // the bloom index is never built into fixed scratch
+    BT_TRY(bt_prep_priority(dg, fixed));
     BT_TRY(tm.start(1));
     constexpr unsigned T = 256;
     k_root_w<<<grid_for(uint64_t(n) * kWave, T, 1u << 30), T, 0, s>>>(n, dg->r_off, dg->r_lo, dg->r_p, dg->r_w,
@@ -464,9 +474,18 @@ int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total_out, uint64_t* s
         size_t const pool_entries_max = size_t(256) << 20;  // 1 GiB of u32 counters
         size_t max_table = 0;
         for (uint32_t r : hroots) max_table = std::max<size_t>(max_table, r);
-        size_t const pool_entries = std::max(max_table, std::min<size_t>(pool_entries_max, size_t(n) * nc));
-        BT_TRY(bt_buf_reserve(c->pool, pool_entries * sizeof(uint32_t)));
-        uint32_t* pool = static_cast<uint32_t*>(c->pool.ptr);
+        size_t pool_entries = std::max(max_table, std::min<size_t>(pool_entries_max, size_t(n) * nc));
+        uint32_t* pool = nullptr;
+        if (fixed) {
+            // the sort storage is done with (stream order): the tables take
+            // the whole region, at least the largest root's table
+            pool_entries = fixed->bytes / sizeof(uint32_t);
+            if (max_table > pool_entries) return BT_ERR_SCRATCH;
+            pool = static_cast<uint32_t*>(fixed->ptr);
+        } else {
+            BT_TRY(bt_buf_reserve(c->pool, pool_entries * sizeof(uint32_t)));
+            pool = static_cast<uint32_t*>(c->pool.ptr);
+        }
         HubItem* ditems = reinterpret_cast<HubItem*>(dg->r_key);  // scratch: 2m+n u64 >= items
         size_t const items_cap = (size_t(2) * m + n) * sizeof(uint64_t) / sizeof(HubItem);
         size_t k = 0;
@@ -527,5 +546,55 @@ int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total_out, uint64_t* s
     c->last.count_kernel_ms = tm.ms(2, 3);
     c->last.count_ms = tm.ms(1, 4);
     c->last.total_ms = tm.ms(0, 4);
+    return BT_OK;
+}
+
+// ---- allocation-free ebfc ---------------------------------------------------
+//
+// For callers that cannot allocate device memory: bt_ebfc_static takes the
+// whole scratch of the call (sort storage, then the hub roots' tables) from
+// this array. Each device holds its own copy for the life of the process;
+// host code uses the address hipGetSymbolAddress returns on that device,
+// never the array's name (a host-side stand-in, not a device address).
+static __device__ __align__(256) uint8_t scratch[BT_STATIC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxStaticDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxStaticDevices] = {};
+
+// The array's address on the context's device (looked up once per device,
+// with that device current).
+int static_scratch(bt_ctx* c, void** out) {
+    if (c->device < 0 || c->device >= kMaxStaticDevices) return BT_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[c->device]) {
+        BT_HIP_TRY(hipSetDevice(c->device));
+        void* p = nullptr;
+        BT_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return BT_ERR_DEVICE;
+        g_scratch_dev[c->device] = p;
+    }
+    *out = g_scratch_dev[c->device];
+    return BT_OK;
+}
+}  // namespace
+
+extern "C" int bt_ebfc_static(bt_ctx* ctx, bt_dgraph* dg, uint64_t* support_out, uint64_t* total,
+                              uint64_t* support_max) {
+    if (!ctx || !dg || dg->ctx != ctx) return BT_ERR_INVALID;
+    void* p = nullptr;
+    BT_TRY(static_scratch(ctx, &p));
+    BT_HIP_TRY(hipSetDevice(ctx->device));
+    bt_fixed_scratch const fx{p, size_t(BT_STATIC_SCRATCH_BYTES)};
+    uint64_t t = 0, sm = 0;
+    BT_TRY(bt_count_impl(dg, true, &t, &sm, false, &fx));
+    if (support_out && dg->m) {
+        BT_HIP_TRY(hipMemcpyAsync(support_out, dg->sup, sizeof(uint64_t) * dg->m, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+        BT_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    if (total) *total = t;
+    if (support_max) *support_max = sm;
     return BT_OK;
 }

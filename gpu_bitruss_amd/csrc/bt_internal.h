@@ -173,10 +173,20 @@ struct bt_timer {
     double ms(int a, int b);
 };
 
+// A fixed device region that a call takes its scratch from instead of the
+// context's growable buffers (the allocation-free entry point).
+struct bt_fixed_scratch {
+    void* ptr;
+    size_t bytes;
+};
+
 // Algorithm entry points implemented in the .hip files (inputs checked by
-// the C-ABI layer).
-int bt_prep_priority(bt_dgraph* dg);
-int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total, uint64_t* support_max, bool emit = false);
+// the C-ABI layer). With `fixed`, the counting's scratch (sort storage, hub
+// tables) comes from that region, nothing is allocated, and a graph that
+// needs more returns BT_ERR_SCRATCH.
+int bt_prep_priority(bt_dgraph* dg, const bt_fixed_scratch* fixed = nullptr);
+int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total, uint64_t* support_max, bool emit = false,
+                  const bt_fixed_scratch* fixed = nullptr);
 // inc_records: incidences name the record holding the edge instead of the
 // bloom's first record (the h-index rounds need the edge's middle)
 int bt_index_build(bt_dgraph* dg, bool inc_records = false);

@@ -157,6 +157,20 @@ int bt_bfc(bt_ctx* ctx, bt_dgraph* dg, uint64_t* total);
 int bt_ebfc(bt_ctx* ctx, bt_dgraph* dg, uint64_t* support_out, uint64_t* total,
             uint64_t* support_max);
 
+/* Allocation-free per-edge support. No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Same results as
+ * bt_ebfc on an uploaded graph, but the call's scratch (sort storage and
+ * hub-root count tables) is a fixed device array of BT_STATIC_SCRATCH_BYTES
+ * per device, held from the first call until the process ends, and nothing
+ * is allocated. A graph that needs more (roughly m above 5 M edges: the sort
+ * storage takes about 24 B per edge) returns BT_ERR_SCRATCH. Every call on
+ * one device shares that array: calls that overlap on one device (from
+ * several threads or contexts) race and must be serialised by the caller. */
+#define BT_STATIC_SCRATCH_BYTES (128u << 20)
+int bt_ebfc_static(bt_ctx* ctx, bt_dgraph* dg, uint64_t* support_out, uint64_t* total,
+                   uint64_t* support_max);
+
 /* h-index bitruss decomposition. Replaces g_bitruss_hindex
  * (bitruss_hidx.cu:179-256), which writes g->bitruss. bitruss_out (m
  * entries) may be NULL; rounds (may be NULL) receives the number of
