@@ -331,7 +331,7 @@ static inline void attribute(peel_ctx* c, uint32_t uv, uint32_t wv, uint32_t wx,
  * edges hold their result, the others their current support); *complete
  * says whether every edge was peeled. */
 static int msp_body(const or_graph* g, int threads, uint64_t* bitruss, uint32_t* levels_out,
-                    or_stats* stats, double max_seconds, uint64_t max_level, int* complete) {
+                    or_stats* stats, double max_seconds, uint64_t max_level, int* complete, int baseline) {
     if (check_graph(g) || (g->m && !bitruss)) return OR_EINVAL;
     const double t_start = omp_get_wtime();
     const uint32_t m = g->m, n = g->n;
@@ -390,7 +390,15 @@ static int msp_body(const or_graph* g, int threads, uint64_t* bitruss, uint32_t*
                 for (int64_t ci = 0; ci < (int64_t)curr_len; ci++) {
                     const uint32_t uv = curr[ci];
                     if (processed[uv]) continue;
-                    const uint32_t u = g->edges[2ull * uv], v = g->edges[2ull * uv + 1];
+                    uint32_t u = g->edges[2ull * uv], v = g->edges[2ull * uv + 1];
+                    /* bitruss_mspp's process_edge (bitruss_msp.cu:366-378):
+                     * the pivot is turned around when its upper endpoint's
+                     * degree is above ten times its lower one's */
+                    if (baseline && g->degrees[u] > 10ull * g->degrees[v]) {
+                        const uint32_t t = u;
+                        u = v;
+                        v = t;
+                    }
                     const uint32_t* un = nbr + off[u];
                     const uint32_t ulen = deg[u];
                     for (uint32_t x = 0; x < deg[v]; x++) {
@@ -436,8 +444,9 @@ static int msp_body(const or_graph* g, int threads, uint64_t* bitruss, uint32_t*
             curr_len = next_len;
         }
         /* compaction trigger (:579); affects speed only: it drops edges
-         * whose support is <= level, which are all processed by now */
-        if ((double)edge_num <= (double)mcur * 0.5 && (double)mcur >= (double)m * 0.1) {
+         * whose support is <= level, which are all processed by now
+         * (bitruss_mspp runs without it) */
+        if (!baseline && (double)edge_num <= (double)mcur * 0.5 && (double)mcur >= (double)m * 0.1) {
             uint64_t kept = 0;
             for (uint32_t x = 0; x < n; x++) {
                 uint32_t k = 0;
@@ -475,18 +484,22 @@ static int msp_body(const or_graph* g, int threads, uint64_t* bitruss, uint32_t*
 
 int or_msp(const or_graph* g, int threads, uint64_t* bitruss, uint32_t* levels_out,
            or_stats* stats) {
-    return msp_body(g, threads, bitruss, levels_out, stats, 0.0, 0, NULL);
+    return msp_body(g, threads, bitruss, levels_out, stats, 0.0, 0, NULL, 0);
+}
+
+int or_mspp(const or_graph* g, int threads, uint64_t* bitruss, uint32_t* levels_out, or_stats* stats) {
+    return msp_body(g, threads, bitruss, levels_out, stats, 0.0, 0, NULL, 1);
 }
 
 int or_msp_sample(const or_graph* g, int threads, double max_seconds, uint64_t* work,
                   uint32_t* levels_done, int* complete, or_stats* stats) {
-    return msp_body(g, threads, work, levels_done, stats, max_seconds, 0, complete);
+    return msp_body(g, threads, work, levels_done, stats, max_seconds, 0, complete, 0);
 }
 
 int or_msp_levels(const or_graph* g, int threads, uint64_t max_level, uint64_t* state, uint32_t* levels_done,
                   int* complete, or_stats* stats) {
     if (max_level == 0) return OR_EINVAL;
-    return msp_body(g, threads, state, levels_done, stats, 0.0, max_level, complete);
+    return msp_body(g, threads, state, levels_done, stats, 0.0, max_level, complete, 0);
 }
 
 /* ------------------------------------------------------- textbook peel */

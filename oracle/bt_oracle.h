@@ -11,6 +11,7 @@
  *   or_bfc       src/bfc/cpu_bfc.cpp:137-203
  *   or_hidx      src/bitruss/cpu_bitruss.cpp:125-239
  *   or_msp       src/bitruss/cpu_bitruss.cpp:241-597
+ *   or_mspp      src/bitruss/bitruss_msp.cu:366-378, 600-722 (baseline msp)
  *   or_peel_seq  src/bitruss/cpu_bitruss.cpp:34-122 (textbook peel)
  */
 #ifndef BT_ORACLE_H
@@ -52,6 +53,11 @@ int or_hidx(const or_graph* g, int threads, uint64_t* bitruss, uint32_t* rounds,
             or_stats* stats);
 int or_msp(const or_graph* g, int threads, uint64_t* bitruss, uint32_t* levels,
            or_stats* stats);
+/* bitruss_mspp (bitruss_msp.cu:600-722): or_msp with each pivot taken the
+ * other way round when deg(upper) > 10 deg(lower) (process_edge, :366-378)
+ * and without the compaction */
+int or_mspp(const or_graph* g, int threads, uint64_t* bitruss, uint32_t* levels,
+            or_stats* stats);
 /* The msp computation cut short for timing: support counting, then peel
  * levels until the first level that ends past max_seconds. `work` (m)
  * holds the supports as they stand; *complete is 1 when every edge got
@@ -64,6 +70,28 @@ int or_msp_sample(const or_graph* g, int threads, double max_seconds, uint64_t* 
 int or_msp_levels(const or_graph* g, int threads, uint64_t max_level, uint64_t* state, uint32_t* levels_done,
                   int* complete, or_stats* stats);
 int or_peel_seq(const or_graph* g, uint64_t* bitruss);
+/* msp over butterfly blooms (bt_oracle_bloom.c): the results, level count
+ * and work counters of or_msp, for graphs on which the reference's
+ * per-pivot merges take hours. max_level > 0 stops after that level with
+ * the prefix state of or_msp_levels. flags: OR_BLOOM_SORTED sums the
+ * 3-frontier-edge rule from sorted id lists wherever it applies (a test
+ * switch); OR_BLOOM_EXACT decrements the single survivor always, which is
+ * the textbook peel (the results of or_peel_seq: true bitruss numbers);
+ * OR_BLOOM_PLANT_SWAP puts a known slip into the rule (p and q exchanged)
+ * so that tests can show the comparison with or_msp catches it. */
+#define OR_BLOOM_SORTED 1
+#define OR_BLOOM_EXACT 2
+#define OR_BLOOM_PLANT_SWAP 4
+int or_msp_bloom(const or_graph* g, int threads, uint64_t max_level, int flags, uint64_t* state,
+                 uint32_t* levels_out, int* complete, or_stats* stats);
+/* The same with checkpoints: for each ck_levels[i] (ascending) ck is called
+ * once with the state after every level up to it (the prefix state of
+ * or_msp_levels with max_level = ck_levels[i]), so one long run records
+ * prefixes at several depths. */
+typedef void (*or_checkpoint_fn)(uint64_t level, const uint64_t* state, uint32_t m, void* user);
+int or_msp_bloom_ck(const or_graph* g, int threads, uint64_t max_level, int flags, uint64_t* state,
+                    uint32_t* levels_out, int* complete, or_stats* stats, const uint64_t* ck_levels, uint32_t n_ck,
+                    or_checkpoint_fn ck, void* ck_user);
 
 #ifdef __cplusplus
 }

@@ -60,7 +60,14 @@ def lib() -> C.CDLL:
         L.or_msp_levels.argtypes = [C.POINTER(OrGraph), C.c_int, C.c_uint64, u64p, C.POINTER(C.c_uint32),
                                     C.POINTER(C.c_int), C.POINTER(OrStats)]
         L.or_peel_seq.argtypes = [C.POINTER(OrGraph), u64p]
-        for f in (L.or_ebfc, L.or_bfc, L.or_hidx, L.or_msp, L.or_msp_sample, L.or_msp_levels, L.or_peel_seq):
+        L.or_msp_bloom_ck.argtypes = [C.POINTER(OrGraph), C.c_int, C.c_uint64, C.c_int, u64p, C.POINTER(C.c_uint32),
+                                      C.POINTER(C.c_int), C.POINTER(OrStats), u64p, C.c_uint32, CK_FN, C.c_void_p]
+        L.or_msp_bloom_ck.restype = C.c_int
+        L.or_mspp.argtypes = [C.POINTER(OrGraph), C.c_int, u64p, C.POINTER(C.c_uint32), C.POINTER(OrStats)]
+        L.or_msp_bloom.argtypes = [C.POINTER(OrGraph), C.c_int, C.c_uint64, C.c_int, u64p, C.POINTER(C.c_uint32),
+                                   C.POINTER(C.c_int), C.POINTER(OrStats)]
+        for f in (L.or_ebfc, L.or_bfc, L.or_hidx, L.or_msp, L.or_msp_sample, L.or_msp_levels, L.or_peel_seq,
+                  L.or_msp_bloom, L.or_mspp):
             f.restype = C.c_int
         _lib = L
     return _lib
@@ -117,6 +124,15 @@ def msp(g, threads: int = 1):
     return out[: g.m], levels.value, st.as_dict()
 
 
+def mspp(g, threads: int = 1):
+    """(bitruss[m] uint64, levels, stats) — bitruss_mspp, bitruss_msp.cu:600-722."""
+    v = _View(g)
+    out = np.zeros(max(g.m, 1), np.uint64)
+    levels, st = C.c_uint32(), OrStats()
+    _run("mspp", lib().or_mspp(C.byref(v.s), threads, out.ctypes.data_as(u64p), C.byref(levels), C.byref(st)))
+    return out[: g.m], levels.value, st.as_dict()
+
+
 def msp_sample(g, threads: int, max_seconds: float):
     """msp cut short for timing: support counting, then peel levels until
     the first one that ends past max_seconds. Returns (levels_done,
@@ -146,3 +162,42 @@ def peel_seq(g):
     out = np.zeros(max(g.m, 1), np.uint64)
     _run("peel_seq", lib().or_peel_seq(C.byref(v.s), out.ctypes.data_as(u64p)))
     return out[: g.m]
+
+
+BLOOM_SORTED, BLOOM_EXACT, BLOOM_PLANT_SWAP = 1, 2, 4
+CK_FN = C.CFUNCTYPE(None, C.c_uint64, u64p, C.c_uint32, C.c_void_p)
+
+
+def msp_bloom_checkpoints(g, threads: int, levels, on_state, flags: int = 0):
+    """or_msp_bloom's whole decomposition, calling on_state(level, state)
+    with the prefix state after every level up to each of `levels`
+    (ascending; state is a numpy view valid during the call only). Returns
+    (result[m], levels, complete, stats)."""
+    v = _View(g)
+    out = np.zeros(max(g.m, 1), np.uint64)
+    lv, done, st = C.c_uint32(), C.c_int(), OrStats()
+    cks = np.ascontiguousarray(sorted(levels), dtype=np.uint64)
+
+    def cb(level, ptr, m, _user):
+        on_state(int(level), np.ctypeslib.as_array(ptr, shape=(max(int(m), 1),))[: int(m)])
+
+    fn = CK_FN(cb)
+    _run("msp_bloom_ck", lib().or_msp_bloom_ck(C.byref(v.s), threads, 0, int(flags), out.ctypes.data_as(u64p),
+                                               C.byref(lv), C.byref(done), C.byref(st), cks.ctypes.data_as(u64p),
+                                               len(cks), fn, None))
+    return out[: g.m], lv.value, bool(done.value), st.as_dict()
+
+
+def msp_bloom(g, threads: int = 1, max_level: int = 0, flags: int = 0):
+    """msp over butterfly blooms (bt_oracle_bloom.c): (state[m] uint64,
+    levels, complete, stats). With max_level = 0 the state is the bitruss
+    result of or_msp; otherwise the prefix state of or_msp_levels. flags:
+    BLOOM_SORTED forces the sorted-list form of the rule; BLOOM_EXACT gives
+    the textbook peel (true bitruss numbers, as peel_seq and hidx);
+    BLOOM_PLANT_SWAP puts a known slip into the rule (tests only)."""
+    v = _View(g)
+    out = np.zeros(max(g.m, 1), np.uint64)
+    levels, done, st = C.c_uint32(), C.c_int(), OrStats()
+    _run("msp_bloom", lib().or_msp_bloom(C.byref(v.s), threads, max_level, int(flags),
+                                         out.ctypes.data_as(u64p), C.byref(levels), C.byref(done), C.byref(st)))
+    return out[: g.m], levels.value, bool(done.value), st.as_dict()

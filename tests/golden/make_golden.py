@@ -27,6 +27,15 @@ A decomposition too long for the oracle is recorded by a prefix instead:
 result, the others their current support):
 
     python tests/golden/make_golden.py skewed_60m --skip-msp --skip-hidx --msp-prefix 1000
+
+--msp-bloom records msp with the bloom restatement (or_msp_bloom, pinned
+against or_msp on the graphs the latter finishes and on the prefix above)
+instead of the per-pivot oracle, whose merges take hours on the 60M-edge
+graph:
+
+    python tests/golden/make_golden.py skewed_60m --skip-hidx --msp-bloom
+
+Fields of an existing file that a run does not recompute are kept.
 """
 from __future__ import annotations
 
@@ -57,6 +66,8 @@ def main() -> None:
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 4)
     ap.add_argument("--skip-hidx", action="store_true")
     ap.add_argument("--skip-msp", action="store_true")
+    ap.add_argument("--msp-bloom", action="store_true",
+                    help="msp by the bloom restatement (or_msp_bloom) instead of or_msp")
     ap.add_argument("--msp-prefix", type=int, default=0, metavar="L",
                     help="also record the msp state after levels 1..L")
     ap.add_argument("--msp-from", nargs=2, metavar=("JSON", "U64"),
@@ -69,15 +80,25 @@ def main() -> None:
     for wl in args.workloads:
         g = gb.Graph.generate(wl)
         sup, total, smax, st = oracle.ebfc(g, args.threads)
-        out = {
+        path = os.path.join(HERE, f"{wl}.json")
+        out = {}
+        if os.path.exists(path):
+            with open(path) as f:
+                out = json.load(f)
+        out.update({
             "workload": wl, "m": g.m, "n": g.n, "u_num": g.u_num, "l_num": g.l_num,
             "neighbors_sha256": hashlib.sha256(g.neighbors.tobytes()).hexdigest(),
             "butterflies": total, "support_max": smax, "ebfc_sha256": digest(sup),
             "w_cnt": st["w_cnt"],
-        }
-        if not args.skip_msp:
+        })
+        if not args.skip_msp and args.msp_bloom:
+            res, levels, complete, st2 = oracle.msp_bloom(g, args.threads)
+            assert complete
+            out.update({"msp_levels": levels, "msp_sha256": digest(res), "msp_stats": st2,
+                        "msp_oracle": "or_msp_bloom"})
+        elif not args.skip_msp:
             res, levels, st2 = oracle.msp(g, args.threads)
-            out.update({"msp_levels": levels, "msp_sha256": digest(res), "msp_stats": st2})
+            out.update({"msp_levels": levels, "msp_sha256": digest(res), "msp_stats": st2, "msp_oracle": "or_msp"})
         if args.msp_prefix:
             st, _, st4 = oracle.msp_levels(g, args.threads, args.msp_prefix)
             out.update({"msp_prefix_levels": args.msp_prefix, "msp_prefix_sha256": digest(st),
@@ -102,7 +123,6 @@ def main() -> None:
             out.update({"hidx_sha256": digest(h), "hidx_kmax": int(h.max()) if g.m else 0,
                         "hidx_rounds_jacobi": run["iters"],
                         "hidx_stats": {k: run[k] for k in ("w_h", "i_h", "b_h")}})
-        path = os.path.join(HERE, f"{wl}.json")
         with open(path, "w") as f:
             json.dump(out, f, indent=1, sort_keys=True)
         print("wrote", path)

@@ -92,14 +92,20 @@ def textbook_peel(g):
     return out
 
 
-def msp_by_butterflies(g, variant: str = "", max_level: int = 0):
+def msp_by_butterflies(g, variant: str = "", max_level: int = 0, turned=frozenset()):
     """(result list, levels) of the reference msp, butterfly-centric.
 
     `variant` plants a deliberate mistake for the tests that show the pins
     can see one: "always" decrements every single survivor, "swap_pq"
     exchanges the roles of the two frontier edges adjacent to it.
     `max_level` > 0 stops before the first level above it: the list then
-    holds the peeled edges' results and the other edges' supports."""
+    holds the peeled edges' results and the other edges' supports.
+    `turned` holds the edges that bitruss_mspp takes as pivots the other way
+    round (process_edge, bitruss_msp.cu:366-378). Only the pivot opposite
+    the survivor sees its orientation: its literal condition names the
+    survivor's neighbours by the pivot's own (u, v), so turning it around
+    exchanges p and q in (o<q<p); the adjacent pivots' conditions read
+    pivot < other adjacent < opposite either way."""
     _, sup = brute_support(g)
     sup = list(sup)
     bfs = butterflies(g)
@@ -136,7 +142,8 @@ def msp_by_butterflies(g, variant: str = "", max_level: int = 0):
                     o, p, q = opposite[f], same_upper[f], same_lower[f]
                     if variant == "swap_pq":
                         p, q = q, p
-                    if variant == "always" or o < q < p or p < o < q or q < o < p:
+                    first = (o < p < q) if o in turned else (o < q < p)
+                    if variant == "always" or first or p < o < q or q < o < p:
                         dec[f] += 1
             nxt = set()
             for f, k in dec.items():
@@ -148,3 +155,9 @@ def msp_by_butterflies(g, variant: str = "", max_level: int = 0):
                 processed[e] = True
             curr = nxt
     return sup, level_max
+
+
+def turned_edges(g):
+    """Edges bitruss_mspp takes the other way round: deg(upper) > 10 deg(lower)."""
+    deg = [int(x) for x in g.degrees]
+    return frozenset(e for e in range(g.m) if deg[int(g.edges[2 * e])] > 10 * deg[int(g.edges[2 * e + 1])])

@@ -150,3 +150,40 @@ def test_work_counters_on_aw():
     # priority wedges of aw.graph, enumerated by hand: root 6 (degree 5)
     # has 5 (via 0, 1, 2, 2, 4), root 5 has 1 (5-2-7), root 2 has 1 (2-7-4)
     assert st["w_cnt"] == 7
+
+
+def test_turn_graph_known_answers():
+    """The hand-derived turn graph (tests/graphs.py): msp decrements e28 in
+    {0,1}x{3,4}, bitruss_mspp does not (its pivot e0 is turned around), and
+    the true bitruss numbers are msp's here."""
+    g = G.from_pairs(G.TURN_EDGES)
+    assert g.m == 32
+    assert oracle.msp(g)[0].tolist() == G.TURN_MSP
+    r, levels, _ = oracle.mspp(g)
+    assert r.tolist() == G.TURN_MSPP and levels == 2
+    assert oracle.peel_seq(g).tolist() == G.TURN_MSP
+    assert refimpl.msp_by_butterflies(g, turned=refimpl.turned_edges(g))[0] == G.TURN_MSPP
+
+
+@pytest.mark.parametrize("seed", list(range(6)))
+def test_mspp_matches_butterfly_centric_rule(seed):
+    """or_mspp against the butterfly-centric restatement with the turned
+    pivots (derived, not retyped: only the pivot opposite the survivor sees
+    its orientation) on graphs with a hub of turned edges, and quirk seeds."""
+    for g in (G.turn_graph(seed), G.quirk_graph(G.QUIRK_SEEDS[seed % len(G.QUIRK_SEEDS)])):
+        want, wl = refimpl.msp_by_butterflies(g, turned=refimpl.turned_edges(g))
+        got, levels, _ = oracle.mspp(g, 2)
+        assert got.tolist() == want and levels == wl
+
+
+def test_mspp_equals_msp_without_turned_edges():
+    for g in (G.from_pairs(G.AW_EDGES), G.from_pairs(G.QUIRK7_EDGES), G.complete(4, 6),
+              G.random_bipartite(3, 20, 20, 0.3)):
+        assert not refimpl.turned_edges(g)
+        assert np.array_equal(oracle.mspp(g, 2)[0], oracle.msp(g, 2)[0])
+
+
+def test_turn_slip_is_caught():
+    """Ignoring the orientation (msp's rule) is a slip the turn graph shows."""
+    g = G.from_pairs(G.TURN_EDGES)
+    assert refimpl.msp_by_butterflies(g)[0] != oracle.mspp(g)[0].tolist()

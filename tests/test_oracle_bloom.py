@@ -1,0 +1,110 @@
+"""Pins the bloom restatement of msp (oracle/bt_oracle_bloom.c), the
+checker for graphs on which the reference's per-pivot merges (or_msp) take
+hours: it must reproduce or_msp — results, level count and every work
+counter of the reference loops — and or_msp_levels' prefix states, and a
+planted slip in its attribution rule must show."""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import gpu_bitruss_amd as gb
+from oracle import oracle
+from tests import graphs as G
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+COUNTERS = ("w_peel", "i_peel", "b_peel", "d_peel", "sum_curr", "sub_rounds")
+
+
+def cases():
+    out = [("aw", G.from_pairs(G.AW_EDGES)), ("quirk7", G.from_pairs(G.QUIRK7_EDGES)),
+           ("K5x7", G.complete(5, 7)), ("K2x2", G.complete(2, 2))]
+    out += [(f"quirk{s}", G.quirk_graph(s)) for s in G.QUIRK_SEEDS + G.ORDER_SEEDS]
+    out += [(f"random{s}", G.random_bipartite(s, 30, 40, 0.3)) for s in range(8)]
+    out += [("dense", G.random_bipartite(99, 25, 25, 0.8)), ("hub", G.hub_graph(5, 6000, 20000, 400, 7000))]
+    out += [(f"mix{i}", gb.Graph.generate("mix_graph", i)) for i in (0, 1, 2, 17)]
+    return out
+
+
+CASES = cases()
+
+
+@pytest.mark.parametrize("name,g", CASES, ids=[c[0] for c in CASES])
+def test_bloom_equals_per_pivot_msp(name, g):
+    want, wlevels, wst = oracle.msp(g, 4)
+    for flags in (0, oracle.BLOOM_SORTED):
+        got, levels, complete, st = oracle.msp_bloom(g, 4, 0, flags)
+        assert np.array_equal(got, want) and levels == wlevels and complete, (name, flags)
+        assert {k: st[k] for k in COUNTERS} == {k: wst[k] for k in COUNTERS}, (name, flags)
+    if name in ("aw", "quirk7"):
+        assert got.tolist() == (G.AW_BITRUSS if name == "aw" else G.QUIRK7_MSP)
+
+
+def test_bloom_results_independent_of_thread_count():
+    g = gb.Graph.generate("mix_graph", 17)
+    a = oracle.msp_bloom(g, 1)
+    b = oracle.msp_bloom(g, 7)
+    assert np.array_equal(a[0], b[0]) and a[1] == b[1] and a[3] == b[3]
+
+
+@pytest.mark.parametrize("name,g", [c for c in CASES if c[0].startswith(("quirk", "mix", "hub"))],
+                         ids=[c[0] for c in CASES if c[0].startswith(("quirk", "mix", "hub"))])
+def test_bloom_prefix_equals_level_bounded_msp(name, g):
+    _, top, _ = oracle.msp(g, 4)
+    for bound in sorted({1, 2, 3, max(1, top // 2), top, top + 2}):
+        want, wcomplete, _ = oracle.msp_levels(g, 4, bound)
+        got, levels, complete, _ = oracle.msp_bloom(g, 4, bound)
+        assert np.array_equal(got, want) and complete == wcomplete, (name, bound)
+
+
+@pytest.mark.parametrize("plant", [oracle.BLOOM_EXACT, oracle.BLOOM_PLANT_SWAP])
+def test_planted_rule_slip_is_caught(plant):
+    """The comparison with or_msp can see a plausible slip in the
+    three-frontier-edge rule (the decrement always taken; p and q
+    exchanged): some quirk seed disagrees."""
+    caught = False
+    for s in G.QUIRK_SEEDS + G.ORDER_SEEDS:
+        g = G.quirk_graph(s)
+        if not np.array_equal(oracle.msp_bloom(g, 2, 0, plant)[0], oracle.msp(g, 2)[0]):
+            caught = True
+            break
+    assert caught
+
+
+def test_bloom_on_powerlaw_2m_golden():
+    """The 2M-edge graph: the digest, level count and every counter that the
+    per-pivot oracle recorded in the golden file."""
+    gold = json.load(open(os.path.join(GOLDEN, "powerlaw_2m.json")))
+    g = gb.Graph.generate("powerlaw_2m")
+    res, levels, complete, st = oracle.msp_bloom(g, os.cpu_count() or 4)
+    assert complete and levels == gold["msp_levels"]
+    assert hashlib.sha256(np.ascontiguousarray(res, dtype="<u8").tobytes()).hexdigest() == gold["msp_sha256"]
+    assert {k: st[k] for k in COUNTERS} == {k: gold["msp_stats"][k] for k in COUNTERS}
+
+
+@pytest.mark.parametrize("name,g", [c for c in CASES if not c[0].startswith("mix")] +
+                         [("turn", G.from_pairs(G.TURN_EDGES))],
+                         ids=[c[0] for c in CASES if not c[0].startswith("mix")] + ["turn"])
+def test_exact_mode_equals_textbook_peel(name, g):
+    """BLOOM_EXACT (a destroyed butterfly costs each survivor 1) gives the
+    true bitruss numbers: the sequential textbook peel and the h-index
+    fixpoint agree with it."""
+    want = oracle.peel_seq(g)
+    got, levels, complete, _ = oracle.msp_bloom(g, 4, 0, oracle.BLOOM_EXACT)
+    assert np.array_equal(got, want) and complete, name
+    assert np.array_equal(oracle.hidx(g, 4)[0], want), name
+    assert levels == (int(want.max()) if g.m else 0)
+
+
+def test_exact_mode_on_powerlaw_2m_golden():
+    """The true bitruss numbers of the 2M-edge graph: the h-index digest of
+    the golden file (the per-pivot oracle's Jacobi rounds)."""
+    gold = json.load(open(os.path.join(GOLDEN, "powerlaw_2m.json")))
+    g = gb.Graph.generate("powerlaw_2m")
+    res, levels, complete, _ = oracle.msp_bloom(g, os.cpu_count() or 4, 0, oracle.BLOOM_EXACT)
+    assert complete and levels == gold["hidx_kmax"]
+    assert hashlib.sha256(np.ascontiguousarray(res, dtype="<u8").tobytes()).hexdigest() == gold["hidx_sha256"]
