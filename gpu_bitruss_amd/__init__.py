@@ -12,6 +12,8 @@ from .device import (
     bfc_evpp,
     bfc_vpp,
     bitruss_msp,
+    bitruss_mspp,
+    bitruss_peeling,
     default_device,
     device_count,
     g_bitruss_hindex,
@@ -20,5 +22,5 @@ from ._lib import BitrussError, LIB_PATH, COH_PATH
 
 __all__ = [
     "Graph", "WORKLOADS", "Batch", "Device", "Server", "DeviceGraph", "bfc_vpp", "bfc_evpp", "g_bitruss_hindex",
-    "bitruss_msp", "default_device", "device_count", "BitrussError", "LIB_PATH", "COH_PATH",
+    "bitruss_msp", "bitruss_mspp", "bitruss_peeling", "default_device", "device_count", "BitrussError", "LIB_PATH", "COH_PATH",
 ]

@@ -171,6 +171,8 @@ def lib() -> C.CDLL:
         "bt_ebfc_static": (C.c_int, [vp, vp, u64p, u64p, u64p]),
         "bt_hidx": (C.c_int, [vp, vp, u64p, C.POINTER(C.c_uint32)]),
         "bt_msp": (C.c_int, [vp, vp, u64p, C.POINTER(C.c_uint32)]),
+        "bt_mspp": (C.c_int, [vp, vp, u64p, C.POINTER(C.c_uint32)]),
+        "bt_peel": (C.c_int, [vp, vp, u64p, C.POINTER(C.c_uint32)]),
         "bt_batch_upload": (C.c_int, [vp, C.POINTER(BtHostGraph), C.c_size_t, C.POINTER(vp)]),
         "bt_batch_free": (C.c_int, [vp]),
         "bt_batch_chunks": (C.c_int, [vp, C.POINTER(C.c_size_t)]),

@@ -331,12 +331,13 @@ __global__ __launch_bounds__(kPeelBlock) void k_hidx_round(uint32_t m, Csr g, co
 // msp entry: supports (and, for the index path, bloom records) first, then
 // the peel. The index path needs cooperative launch and room for the index
 // (about 44 bytes per priority wedge); otherwise the merge path runs.
-int bt_msp_impl(bt_dgraph* dg, uint32_t* levels_out) {
+// rule 1 (mspp) and 2 (the textbook peel) run on the index path only.
+int bt_msp_impl(bt_dgraph* dg, uint32_t* levels_out, int rule) {
     bt_ctx* c = dg->ctx;
-    bool const want_index = c->coop && c->msp_path != BT_MSP_PATH_MERGE;
+    bool const want_index = c->coop && (c->msp_path != BT_MSP_PATH_MERGE || rule != 0);
     BT_TRY(bt_count_impl(dg, true, nullptr, nullptr, want_index));
     if (want_index && c->ix.R != kNoIndex) {
-        int st = bt_msp_index_peel(dg, levels_out);
+        int st = bt_msp_index_peel(dg, levels_out, rule);
         if (st != BT_ERR_BUSY) return st;
         // the peel's share of the CUs was held by other work (its grid
         // barrier timed out): once more, from the supports, as one
@@ -344,11 +345,11 @@ int bt_msp_impl(bt_dgraph* dg, uint32_t* levels_out) {
         int const share = c->peel_blocks;
         c->peel_blocks = 0;
         st = bt_count_impl(dg, true, nullptr, nullptr, true);
-        if (st == BT_OK) st = c->ix.R != kNoIndex ? bt_msp_index_peel(dg, levels_out) : BT_ERR_BUDGET;
+        if (st == BT_OK) st = c->ix.R != kNoIndex ? bt_msp_index_peel(dg, levels_out, rule) : BT_ERR_BUDGET;
         c->peel_blocks = share;
         return st == BT_ERR_INTERNAL ? BT_ERR_BUSY : st;
     }
-    if (c->msp_path == BT_MSP_PATH_INDEX) return BT_ERR_BUDGET;
+    if (c->msp_path == BT_MSP_PATH_INDEX || rule != 0) return BT_ERR_BUDGET;
     return bt_msp_merge_peel(dg, levels_out);
 }
 

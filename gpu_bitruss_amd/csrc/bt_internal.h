@@ -82,6 +82,7 @@ struct bt_index {
     bt_buf seg;           // u32 per block + 1: segment offsets, then counts
     bt_buf c2s;           // u32 per peel wave: sorted both-frontier lists
     bt_buf hdirty;        // u8 2 x blooms: h-index rounds' change marks
+    bt_buf flip;          // u32 per 32 edges: mspp's turned-around pivots
     uint64_t R = 0, NB = 0;
 };
 
@@ -190,11 +191,12 @@ int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total, uint64_t* suppo
 // inc_records: incidences name the record holding the edge instead of the
 // bloom's first record (the h-index rounds need the edge's middle)
 int bt_index_build(bt_dgraph* dg, bool inc_records = false);
-int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels);
+// rule: 0 msp, 1 mspp, 2 the textbook (exact) peel
+int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels, int rule = 0);
 int bt_msp_merge_peel(bt_dgraph* dg, uint32_t* levels);
 // ix.R value meaning "records were not emitted" (index path unavailable)
 constexpr uint64_t kNoIndex = ~0ull;
-int bt_msp_impl(bt_dgraph* dg, uint32_t* levels);
+int bt_msp_impl(bt_dgraph* dg, uint32_t* levels, int rule = 0);
 int bt_hidx_impl(bt_dgraph* dg, uint32_t* rounds);
 int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds);
 

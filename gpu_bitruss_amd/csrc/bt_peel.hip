@@ -79,6 +79,19 @@ constexpr uint32_t kBigCap = 64;        // large blooms a block walks together p
 constexpr uint32_t kWindowEdges = 4u << 20;  // live edges from which the collect uses a window
 constexpr int kSp = 32;                 // u32 spacing of the barrier slots (128 B)
 constexpr uint64_t kInf = ~0ull;
+// Attribution of a butterfly with three frontier edges to its survivor f
+// (o the frontier edge opposite f, p the one sharing f's upper vertex, q the
+// one sharing f's lower vertex):
+//   kRuleMsp    bitruss_msp (cpu_bitruss.cpp:301-351): f loses 1 iff
+//               (o<q<p) or (p<o<q) or (q<o<p)
+//   kRuleMspp   bitruss_mspp (bitruss_msp.cu:600-722): the same, except that
+//               process_edge (:366-378) turned the pivot o around (its upper
+//               degree above 10x its lower one), which exchanges p and q in
+//               o's own condition: (o<p<q) or (p<o<q) or (q<o<p)
+//   kRuleExact  the textbook peel (bitruss_peeling_cpu, cpu_bitruss.cpp:
+//               34-122): every destroyed butterfly costs each survivor 1, so
+//               f loses 1; the levels then give the true bitruss numbers
+constexpr uint32_t kRuleMsp = 0, kRuleMspp = 1, kRuleExact = 2;
 constexpr uint64_t kBarrierLimit = 30ull * 100000000ull;  // 30 s of the 100 MHz clock
 constexpr unsigned kRelErr = 0x80000000u;                   // release word: error bit
 constexpr uint32_t kSpan = 1024;        // graphs of a block's segment with an LDS minimum
@@ -156,6 +169,10 @@ struct MspArgs {
     uint32_t window_min;   // live edges from which a split opens a window
     uint64_t max_level;    // plain graphs: stop after this level (0: never)
     uint32_t big_min;      // blooms with more middles are walked by the whole block
+    // attribution rule for a butterfly with three frontier edges (kRule*);
+    // mspp reads the orientation flag of each edge from flipbits
+    uint32_t rule;
+    const uint32_t* flipbits;
 };
 
 // relaxed device-scope accesses (coherent across XCDs)
@@ -502,16 +519,24 @@ __device__ __forceinline__ bool rule3(uint32_t o, uint32_t p, uint32_t q) {
     return (o < q && q < p) || (p < o && o < q) || (q < o && o < p);
 }
 
+__device__ __forceinline__ bool flipped(const MspArgs& a, uint32_t e) {
+    return (a.flipbits[e >> 5] >> (e & 31u)) & 1u;
+}
+
 // Decrement owed by the survivor of a one-frontier-edge middle (g is the
-// frontier edge of the same middle) for the both-frontier middles.
+// frontier edge of the same middle) for the both-frontier middles. Under
+// kRuleMspp a turned-around o exchanges the roles of p and q, which is the
+// rule of the other side.
 __device__ __forceinline__ uint64_t rule_sum(const uint64_t* c2, uint32_t n2, bool froot, uint32_t up,
-                                             uint32_t g) {
+                                             uint32_t g, const MspArgs& a) {
+    if (a.rule == kRuleExact) return n2;
     uint64_t d = 0;
     for (uint32_t t = 0; t < n2; t++) {
         uint64_t const rr = c2[t];
         uint32_t const o = froot ? uint32_t(rr) : uint32_t(rr >> 32);
         uint32_t const sr = froot ? uint32_t(rr >> 32) : uint32_t(rr);
-        d += up ? rule3(o, sr, g) : rule3(o, g, sr);
+        bool const u2 = (a.rule == kRuleMspp && flipped(a, o)) ? !up : bool(up);
+        d += u2 ? rule3(o, sr, g) : rule3(o, g, sr);
     }
     return d;
 }
@@ -631,6 +656,23 @@ struct C2Lists {
         for (int i = 0; i < 4; i++)
             wave_radix_sort(i == 0 ? e1p : i == 1 ? e2p : i == 2 ? e1q : e2q, i < 2 ? np : nq, tmp, hist, bits);
     }
+    // kRuleMspp: the rule pairwise over the unsorted lists (the counts
+    // above assume one orientation for every o)
+    __device__ __forceinline__ uint64_t rule_pairs(bool froot, uint32_t up, uint32_t g, const MspArgs& a) const {
+        uint64_t d = 0;
+        for (int h = 0; h < 2; h++) {
+            const uint32_t* e1 = h ? e1q : e1p;
+            const uint32_t* e2 = h ? e2q : e2p;
+            uint32_t const n = h ? cq : cp;
+            for (uint32_t t = 0; t < n; t++) {
+                uint32_t const o = froot ? e2[t] : e1[t];
+                uint32_t const sr = froot ? e1[t] : e2[t];
+                bool const u2 = flipped(a, o) ? !up : bool(up);
+                d += u2 ? rule3(o, sr, g) : rule3(o, g, sr);
+            }
+        }
+        return d;
+    }
     __device__ __forceinline__ uint64_t rule(bool froot, uint32_t up, uint32_t g) const {
         if (up)
             return froot ? uint64_t(nq - count_below(e1q, nq, g)) + count_below(e2p, np, g)
@@ -689,7 +731,7 @@ __device__ __forceinline__ bool process_bloom(const MspArgs& a, uint32_t s, uint
     uint32_t const dec0 = n1 + n2;
     // more both-frontier middles than the LDS list holds, and middles that
     // need the rule: the streaming path (nothing is written yet)
-    if (n2 > a.c2cap && n1) return false;
+    if (a.rule != kRuleExact && n2 > a.c2cap && n1) return false;
     // decrements: the first target is r-v (c = 0) or the survivor (c = 1),
     // the second is v-w (c = 0 only, by dec0)
     uint32_t t1[RPT], d1[RPT];
@@ -705,7 +747,7 @@ __device__ __forceinline__ bool process_bloom(const MspArgs& a, uint32_t s, uint
             uint32_t const g = froot ? e2 : e1;
             t1[i] = froot ? e1 : e2;
             d1[i] = n0 + (n1 - 1) +
-                    uint32_t(rule_sum(c2, n2, froot, up, g));
+                    uint32_t(rule_sum(c2, n2, froot, up, g, a));
         }
     }
     // the atomics of several rows (8, or 4 in the largest tier) are in
@@ -774,7 +816,7 @@ __device__ __forceinline__ void process_large(const MspArgs& a, uint32_t s, uint
     }
     __builtin_amdgcn_wave_barrier();
     uint64_t const dec0 = uint64_t(n1) + n2;
-    bool const c2ok = n2 <= a.c2cap;
+    bool const c2ok = n2 <= a.c2cap || a.rule == kRuleExact;
     C2Lists lists(wave_scratch(a), np, n2 - np);
     if (!c2ok && n1) {
         for (uint32_t j0 = 0; j0 < kb; j0 += kWave) {
@@ -787,7 +829,8 @@ __device__ __forceinline__ void process_large(const MspArgs& a, uint32_t s, uint
             }
             lists.put(c == 2, r);
         }
-        lists.sort(reinterpret_cast<uint32_t*>(c2), a.id_bits);
+        if (a.rule == kRuleMsp) lists.sort(reinterpret_cast<uint32_t*>(c2), a.id_bits);
+        else scratch_fence();
         if (lane == 0) afetch_add(&a.ctl->sorted, 1ull);
     }
     uint32_t wpos = 0;
@@ -814,7 +857,9 @@ __device__ __forceinline__ void process_large(const MspArgs& a, uint32_t s, uint
             f = froot ? e1 : e2;
             uint32_t const g = froot ? e2 : e1;
             uint64_t const d =
-                uint64_t(n0) + (n1 - 1) + (c2ok ? rule_sum(c2, n2, froot, up, g) : lists.rule(froot, up, g));
+                uint64_t(n0) + (n1 - 1) +
+                (c2ok ? rule_sum(c2, n2, froot, up, g, a)
+                      : a.rule == kRuleMsp ? lists.rule(froot, up, g) : lists.rule_pairs(froot, up, g, a));
             if (d) x1 = dec(f, d);
         }
         // compaction in place: the writes stay below the records read so far
@@ -1115,7 +1160,7 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
             }
             __syncthreads();
             uint32_t const n0 = s_bcnt[0], n1 = s_bcnt[1], n2 = s_bcnt[2];
-            if (n2 > a.c2cap && n1) {
+            if (a.rule != kRuleExact && n2 > a.c2cap && n1) {
                 if (wv == 0) process_large(a, s, kb, up, X, dec, c2, xb, out);
                 __syncthreads();
                 continue;
@@ -1161,7 +1206,7 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
                         bool const froot = s1[u] != X;  // the survivor is r-v
                         uint32_t const g = froot ? e2 : e1;
                         t1[u] = froot ? e1 : e2;
-                        d1[u] = uint64_t(n0) + (n1 - 1) + rule_sum(s_bc2, n2, froot, up, g);
+                        d1[u] = uint64_t(n0) + (n1 - 1) + rule_sum(s_bc2, n2, froot, up, g, a);
                     }
                 }
                 uint64_t o1[UB], o2[UB];
@@ -1600,6 +1645,21 @@ __global__ void k_prefix_state(uint32_t m, const uint32_t* __restrict__ stamp, c
         if (stamp[i] == 0) res[i] = sup[i];
 }
 
+// One bit per edge: the pivot orientation of bitruss_mspp is turned around
+// (deg(upper) > 10 deg(lower), process_edge in bitruss_msp.cu:366-378).
+__global__ void k_orient_bits(uint32_t m, const uint32_t* __restrict__ edges, const uint32_t* __restrict__ deg,
+                              uint32_t* __restrict__ bits) {
+    uint32_t const words = m / 32 + 1;
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < words; w += gridDim.x * blockDim.x) {
+        uint32_t b = 0;
+        for (uint32_t i = 0; i < 32; i++) {
+            uint32_t const e = w * 32 + i;
+            if (e < m && uint64_t(deg[edges[2ull * e]]) > 10ull * deg[edges[2ull * e + 1]]) b |= 1u << i;
+        }
+        bits[w] = b;
+    }
+}
+
 __global__ void k_max_res(uint32_t m, const uint64_t* __restrict__ a, uint64_t* __restrict__ out) {
     uint64_t mx = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
@@ -1612,7 +1672,7 @@ __global__ void k_max_res(uint32_t m, const uint64_t* __restrict__ a, uint64_t* 
 
 // Peels with the bloom index; the supports and index records come from the
 // counting call that precedes it (bt_count_impl with emit).
-int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
+int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out, int rule) {
     bt_ctx* c = dg->ctx;
     hipStream_t s = c->stream;
     uint32_t const m = dg->m;
@@ -1697,6 +1757,17 @@ int bt_msp_index_peel(bt_dgraph* dg, uint32_t* levels_out) {
     // the prefix (peeled edges their result, the others their current
     // support) with BT_PARTIAL; plain graphs only (not batch unions)
     a.max_level = dg->ngraph == 0 ? c->peel_max_level : 0;
+    a.rule = uint32_t(rule);
+    if (rule == kRuleMspp) {
+        // process_edge (bitruss_msp.cu:366-378): an edge whose upper
+        // endpoint's degree is above ten times its lower one's is peeled
+        // with the endpoints exchanged
+        BT_TRY(bt_buf_reserve(ix.flip, sizeof(uint32_t) * (size_t(m) / 32 + 1)));
+        a.flipbits = static_cast<const uint32_t*>(ix.flip.ptr);
+        k_orient_bits<<<grid_for(size_t(m) / 32 + 1, 256, unsigned(c->cu_count) * 8), 256, 0, s>>>(
+            m, dg->edges, dg->deg, static_cast<uint32_t*>(ix.flip.ptr));
+        BT_HIP_TRY(hipGetLastError());
+    }
     a.id_bits = 1;
     while (a.id_bits < 32 && ((m ? m - 1 : 0) >> a.id_bits)) a.id_bits++;
     if (batch) {

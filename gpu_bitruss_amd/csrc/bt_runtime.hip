@@ -238,7 +238,7 @@ int bt_ctx_create(int device, bt_ctx** out) {
     for (bt_buf* b : {&c->sort_tmp, &c->pool, &c->pinned_dummy, &c->ix.keys0, &c->ix.keys1, &c->ix.vals0,
                       &c->ix.vals1, &c->ix.bid, &c->ix.bstart, &c->ix.bk, &c->ix.bclaim, &c->ix.inc_off,
                       &c->ix.inc_cur, &c->ix.inc, &c->ix.touched, &c->ix.area, &c->ix.bgraph, &c->ix.glev,
-                      &c->ix.live, &c->ix.seg, &c->ix.c2s, &c->ix.hdirty})
+                      &c->ix.live, &c->ix.seg, &c->ix.c2s, &c->ix.hdirty, &c->ix.flip})
         b->owner = c;
     if (hipHostMalloc(reinterpret_cast<void**>(&c->h_pinned), 64 * sizeof(uint64_t)) != hipSuccess) {
         bt_ctx_destroy(c);
@@ -257,7 +257,7 @@ int bt_ctx_destroy(bt_ctx* c) {
     bt_buf_free(c->pinned_dummy);
     bt_buf* ixb[] = {&c->ix.keys0, &c->ix.keys1, &c->ix.vals0, &c->ix.vals1, &c->ix.bid, &c->ix.bstart,
                      &c->ix.bk, &c->ix.bclaim, &c->ix.inc_off, &c->ix.inc_cur, &c->ix.inc, &c->ix.touched, &c->ix.area, &c->ix.bgraph, &c->ix.glev,
-                     &c->ix.live, &c->ix.seg, &c->ix.c2s, &c->ix.hdirty};
+                     &c->ix.live, &c->ix.seg, &c->ix.c2s, &c->ix.hdirty, &c->ix.flip};
     for (bt_buf* b : ixb) bt_buf_free(*b);
     for (auto const& kv : c->cache) (void)hipFree(kv.second);
     c->cache.clear();
@@ -328,7 +328,8 @@ int bt_ctx_trim(bt_ctx* ctx) {
     bt_buf* bufs[] = {&ctx->sort_tmp, &ctx->pool, &ctx->ix.keys0, &ctx->ix.keys1, &ctx->ix.vals0, &ctx->ix.vals1,
                       &ctx->ix.bid, &ctx->ix.bstart, &ctx->ix.bk, &ctx->ix.bclaim, &ctx->ix.inc_off,
                       &ctx->ix.inc_cur, &ctx->ix.inc, &ctx->ix.touched, &ctx->ix.area, &ctx->ix.bgraph,
-                      &ctx->ix.glev, &ctx->ix.live, &ctx->ix.seg, &ctx->ix.c2s, &ctx->ix.hdirty};
+                      &ctx->ix.glev, &ctx->ix.live, &ctx->ix.seg, &ctx->ix.c2s, &ctx->ix.hdirty,
+                      &ctx->ix.flip};
     for (bt_buf* b : bufs) bt_buf_free(*b);
     ctx->ix.R = 0;
     ctx->ix.NB = 0;
@@ -445,14 +446,28 @@ int bt_hidx(bt_ctx* ctx, bt_dgraph* dg, uint64_t* bitruss_out, uint32_t* rounds)
     return BT_OK;
 }
 
-int bt_msp(bt_ctx* ctx, bt_dgraph* dg, uint64_t* bitruss_out, uint32_t* levels) {
+namespace {
+int msp_family(bt_ctx* ctx, bt_dgraph* dg, uint64_t* bitruss_out, uint32_t* levels, int rule) {
     BT_TRY(check_graph(ctx, dg));
     uint32_t l = 0;
-    int const st = bt_msp_impl(dg, &l);
+    int const st = bt_msp_impl(dg, &l, rule);
     if (st != BT_OK && st != BT_PARTIAL) return st;
     BT_TRY(copy_out(dg, dg->res, bitruss_out));
     if (levels) *levels = l;
     return st;
+}
+}  // namespace
+
+int bt_msp(bt_ctx* ctx, bt_dgraph* dg, uint64_t* bitruss_out, uint32_t* levels) {
+    return msp_family(ctx, dg, bitruss_out, levels, 0);
+}
+
+int bt_mspp(bt_ctx* ctx, bt_dgraph* dg, uint64_t* bitruss_out, uint32_t* levels) {
+    return msp_family(ctx, dg, bitruss_out, levels, 1);
+}
+
+int bt_peel(bt_ctx* ctx, bt_dgraph* dg, uint64_t* bitruss_out, uint32_t* levels) {
+    return msp_family(ctx, dg, bitruss_out, levels, 2);
 }
 
 }  // extern "C"

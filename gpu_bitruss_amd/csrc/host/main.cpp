@@ -61,7 +61,8 @@ void usage() {
         "  --bin          Output binary file path\n"
         "  --cpu          Run CPU algorithms (not served by this build)\n"
         "  --gpu          Run GPU algorithms\n"
-        "  --algo         Algorithm to run including: bfc, ebfc, hidx, msp [nargs=0..1] [default: \"msp\"]\n"
+        "  --algo         Algorithm to run including: bfc, ebfc, hidx, msp (also mspp, peel) [nargs=0..1] "
+        "[default: \"msp\"]\n"
         "  --threads      Number of threads (must be a positive integer) [nargs=0..1] [default: 1]\n"
         "  --dump         Write the per-edge uint64 result to this file\n"
         "  --json         Print a JSON summary line\n");
@@ -226,7 +227,10 @@ int main(int argc, char** argv) {
     if ((st = bt_graph_upload(ctx, &g, &dg)) != BT_OK) fail("bt_graph_upload", st);
     std::vector<uint64_t> out(g.m ? g.m : 1, 0);
     std::string algo = a.algo;
-    if (algo != "ebfc" && algo != "bfc" && algo != "hidx") algo = "msp";  // main.cu:126-127
+    // main.cu:126-127 sends any other name to msp; mspp and peel are the
+    // reference's two further algorithms (bitruss_mspp, bitruss_peeling_cpu),
+    // which its dispatch does not reach
+    if (algo != "ebfc" && algo != "bfc" && algo != "hidx" && algo != "mspp" && algo != "peel") algo = "msp";
     uint64_t total = 0, smax = 0;
     uint32_t iters = 0;
     bool have_array = true;
@@ -245,6 +249,14 @@ int main(int argc, char** argv) {
         uint64_t kmax = 0;
         for (uint32_t e = 0; e < g.m; e++) kmax = out[e] > kmax ? out[e] : kmax;
         LOG_INFO("k-bitruss max value %llu, with %u rounds", (unsigned long long)kmax, iters);
+    } else if (algo == "mspp") {
+        LOG_INFO("running bitruss decomposition online peeling (baseline)");  // bitruss_msp.cu:603
+        if ((st = bt_mspp(ctx, dg, out.data(), &iters)) != BT_OK) fail("bt_mspp", st);
+        LOG_INFO("(baseline) bitruss msp with %'u levels", iters);  // bitruss_msp.cu:693
+    } else if (algo == "peel") {
+        LOG_INFO("running bitruss decomposition by peeling");
+        if ((st = bt_peel(ctx, dg, out.data(), &iters)) != BT_OK) fail("bt_peel", st);
+        LOG_INFO("k-bitruss max value %u by peeling", iters);
     } else {
         LOG_INFO("running bitruss decomposition online peeling");
         if ((st = bt_msp(ctx, dg, out.data(), &iters)) != BT_OK) fail("bt_msp", st);

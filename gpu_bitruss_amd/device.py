@@ -185,6 +185,20 @@ class DeviceGraph:
             _lib.check("bt_msp", st)
         return (a[: self.m] if a is not None else None), lv.value
 
+    def mspp(self, copy: bool = True):
+        """bt_mspp: the reference's baseline msp (bitruss_mspp)."""
+        a, p = self._out(copy)
+        lv = C.c_uint32()
+        _lib.check("bt_mspp", _lib.lib().bt_mspp(self.dev.handle, self.handle, p, C.byref(lv)))
+        return (a[: self.m] if a is not None else None), lv.value
+
+    def peel(self, copy: bool = True):
+        """bt_peel: true bitruss numbers by peeling (bitruss_peeling_cpu)."""
+        a, p = self._out(copy)
+        lv = C.c_uint32()
+        _lib.check("bt_peel", _lib.lib().bt_peel(self.dev.handle, self.handle, p, C.byref(lv)))
+        return (a[: self.m] if a is not None else None), lv.value
+
     def free(self) -> None:
         if self._h:
             _lib.lib().bt_graph_free(self._h)
@@ -371,3 +385,25 @@ def bitruss_msp(g: Graph, dev: Device | None = None) -> int:
     finally:
         dg.free()
     return levels
+
+
+def bitruss_mspp(g: Graph, dev: Device | None = None) -> int:
+    """The baseline msp (bitruss_msp.cu:600-722) into g.bitruss; returns the
+    number of levels the reference logs."""
+    dg = (dev or default_device()).upload(g)
+    try:
+        g.bitruss, levels = dg.mspp()
+    finally:
+        dg.free()
+    return levels
+
+
+def bitruss_peeling(g: Graph, dev: Device | None = None) -> int:
+    """True bitruss numbers by peeling (the results of bitruss_peeling_cpu,
+    cpu_bitruss.cpp:34-122) into g.bitruss; returns the largest."""
+    dg = (dev or default_device()).upload(g)
+    try:
+        g.bitruss, kmax = dg.peel()
+    finally:
+        dg.free()
+    return kmax

@@ -189,6 +189,23 @@ int bt_hidx(bt_ctx* ctx, bt_dgraph* dg, uint64_t* bitruss_out, uint32_t* rounds)
  * its share and once on the whole device. */
 int bt_msp(bt_ctx* ctx, bt_dgraph* dg, uint64_t* bitruss_out, uint32_t* levels);
 
+/* The reference's baseline msp. Replaces bitruss_mspp
+ * (bitruss_msp.cu:600-722): bt_msp's peel, except that an edge whose upper
+ * endpoint has more than ten times the degree of its lower one is taken as
+ * a pivot the other way round (process_edge, bitruss_msp.cu:366-378), which
+ * changes the attribution of butterflies with three frontier edges. levels
+ * as bt_msp (the reference logs "(baseline) bitruss msp with N levels").
+ * Runs on the bloom index only (BT_ERR_BUDGET when it does not fit). */
+int bt_mspp(bt_ctx* ctx, bt_dgraph* dg, uint64_t* bitruss_out, uint32_t* levels);
+
+/* True bitruss numbers by peeling. Replaces bitruss_peeling_cpu
+ * (cpu_bitruss.cpp:34-122), the reference's sequential textbook peel: the
+ * same level-synchronous peel as bt_msp in which every destroyed butterfly
+ * costs each of its surviving edges exactly 1, which gives the same numbers
+ * as peeling one edge of minimum support at a time (and as bt_hidx).
+ * levels receives the largest bitruss number. Bloom index only. */
+int bt_peel(bt_ctx* ctx, bt_dgraph* dg, uint64_t* bitruss_out, uint32_t* levels);
+
 /* ---- batches: many independent graphs decomposed together --------------
  *
  * No counterpart in the reference, which runs one graph per process. A batch

@@ -49,15 +49,19 @@ def test_coh_gpu_algorithms(tmp_path, dev):
     osup, ototal, osmax, _ = oracle.ebfc(g, 8)
     ohidx, orounds, _ = oracle.hidx(g, 8)
     omsp, olevels, _ = oracle.msp(g, 8)
-    expect = {"ebfc": osup, "hidx": ohidx, "msp": omsp}
+    omspp, opplevels, _ = oracle.mspp(g, 8)
+    expect = {"ebfc": osup, "hidx": ohidx, "msp": omsp, "mspp": omspp, "peel": ohidx}
     wording = {
         "bfc": ["butterfly counting with vertex priority on gpu", f"btf number: {ototal:,}"],
         "ebfc": [f"the max edge support is {osmax:,}", f"total butterfly counting with vertex priority on gpu: {ototal:,}"],
         "hidx": ["running bitruss decomposition based on h-index on GPU",
                  f"k-bitruss max value {int(ohidx.max())}, with {orounds} rounds"],
         "msp": ["running bitruss decomposition online peeling", f"bitruss msp with {olevels:,} levels"],
+        "mspp": ["running bitruss decomposition online peeling (baseline)",
+                 f"(baseline) bitruss msp with {opplevels:,} levels"],
+        "peel": ["running bitruss decomposition by peeling", f"k-bitruss max value {int(ohidx.max())} by peeling"],
     }
-    for algo in ("bfc", "ebfc", "hidx", "msp", "no_such_algo"):
+    for algo in ("bfc", "ebfc", "hidx", "msp", "mspp", "peel", "no_such_algo"):
         say(f"coh --gpu --algo {algo} on {f.name} (m={g.m})")
         dump = tmp_path / f"{algo}.u64"
         rc, out = coh("--gpu", "--algo", algo, "--graph", str(f), "--dump", str(dump), "--json")
@@ -76,6 +80,8 @@ def test_coh_gpu_algorithms(tmp_path, dev):
             assert s["iters"] == orounds
         if used == "msp":
             assert s["iters"] == olevels
+        if used == "mspp":
+            assert s["iters"] == opplevels
         if used in expect:
             got = np.fromfile(str(dump), dtype="<u8")
             assert np.array_equal(got, expect[used]), algo
