@@ -437,7 +437,14 @@ int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total_out, uint64_t* s
         size_t const cap = std::max<uint64_t>(cnts[3], 1);
         size_t free_b = 0, total_b = 0;
         BT_HIP_TRY(hipMemGetInfo(&free_b, &total_b));
-        size_t const have = ix.keys0.bytes + ix.vals0.bytes;
+        // the share is of the memory the index can have: what is free plus
+        // every index buffer the context already holds from an earlier call
+        // (they are reused), so that a repeated call on a large graph finds
+        // the same room as the first
+        size_t const have = ix.keys0.bytes + ix.vals0.bytes + ix.keys1.bytes + ix.vals1.bytes + ix.bid.bytes +
+                            ix.bstart.bytes + ix.bk.bytes + ix.bclaim.bytes + ix.inc_off.bytes + ix.inc_cur.bytes +
+                            ix.inc.bytes + ix.touched.bytes + ix.bgraph.bytes + ix.live.bytes + ix.c2s.bytes +
+                            ix.hdirty.bytes;
         if (double(cap) * 16.0 > (double(free_b) + double(have)) * c->index_mem_frac) emit = false;
     }
     if (emit && per_edge) {

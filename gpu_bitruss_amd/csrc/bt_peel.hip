@@ -898,6 +898,7 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
     __shared__ uint32_t s_big[kBigCap], s_big_bg[kBigCap];
     __shared__ unsigned s_nbig, s_bcnt[4];
     __shared__ uint64_t s_bc2[kC2Cap];
+    __shared__ unsigned long long s_dmid[3];
     uint32_t const G = gridDim.x, B = blockIdx.x, tid = threadIdx.x;
     int const wv = tid / kWave;
     uint32_t const gw = B * kWPB + wv, nw = G * kWPB;
@@ -1002,16 +1003,19 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
     uint64_t W = a.resume ? ctl->W : 0ull;  // top of the collect window (plain graphs)
     uint32_t nt = 0;            // claimed blooms of sub-round X
     uint64_t t_col = 0, t_pe = 0, w_col = 0, w_pe = 0, pmax_sum = 0;
-    uint64_t d_wave = 0, d_block = 0, d_nblock = 0;  // diagnostics: middles walked (lane 0s count)
-    // adds this thread's middle counts to the control block (diagnostics)
+    // diagnostics: middles walked by waves and by the whole block, and the
+    // block walks (in LDS, so that no register holds them)
+    if (tid < 3) s_dmid[tid] = 0;
+    // adds this block's middle counts to the control block (diagnostics)
     auto flush_diag = [&]() {
         if (!a.diag) return;
-        if ((tid & (kWave - 1)) == 0 && d_wave) afetch_add(&ctl->mid_wave, (unsigned long long)d_wave);
-        if (tid == 0 && d_block) {
-            afetch_add(&ctl->mid_block, (unsigned long long)d_block);
-            afetch_add(&ctl->n_block, (unsigned long long)d_nblock);
+        __syncthreads();
+        if (tid == 0) {
+            afetch_add(&ctl->mid_wave, s_dmid[0]);
+            afetch_add(&ctl->mid_block, s_dmid[1]);
+            afetch_add(&ctl->n_block, s_dmid[2]);
+            s_dmid[0] = s_dmid[1] = s_dmid[2] = 0;
         }
-        d_wave = d_block = d_nblock = 0;
     };
 
     // ---- one peel sub-round X over the claimed blooms; level_of(bg) is the
@@ -1087,7 +1091,7 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
                 }
             }
             dec.L = level_of(bg);
-            if (a.diag) d_wave += kb;
+            if (a.diag && lane == 0) atomicAdd(&s_dmid[0], (unsigned long long)kb);
             // a bloom frees at most two edges per middle
             xb.ensure(2 * (kb < uint32_t(16 * kWave) ? kb : uint32_t(16 * kWave)), X + 1, a, out);
             bool done = false;
@@ -1108,16 +1112,16 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
         // live middles; middles keep no order.
         __syncthreads();
         uint32_t const nbig = s_nbig < kBigCap ? s_nbig : kBigCap;
-        constexpr int UB = 4;
+        constexpr int UB = 2;
         for (uint32_t bi = 0; bi < nbig; bi++) {
             uint32_t const s = s_big[bi];
             uint32_t const hdr = ald(&a.hdr[s]);
             uint32_t const kb = hdr & 0x7FFFFFFFu, up = hdr >> 31;
             uint64_t* rec = a.rec + s;
             dec.L = level_of(s_big_bg[bi]);
-            if (a.diag) {
-                d_block += kb;
-                d_nblock++;
+            if (a.diag && tid == 0) {
+                s_dmid[1] += kb;
+                s_dmid[2]++;
             }
             if (tid < 4) s_bcnt[tid] = 0;
             __syncthreads();

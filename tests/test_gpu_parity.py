@@ -373,18 +373,28 @@ def test_skewed_60m_golden(dev):
 
 @pytest.mark.gpu
 def test_skewed_60m_msp_prefix(dev):
-    """The 60M-edge graph's msp decomposition through its first levels
-    (the whole one runs for hours): the state after them — the peeled edges'
-    results and the others' current supports — against the oracle's."""
+    """The 60M-edge graph's msp decomposition through its first levels (the
+    oracles take hours for the whole one): the state after them — the peeled
+    edges' results and the others' current supports — against the per-pivot
+    oracle's (800 levels) and the bloom oracle's deeper checkpoints, on one
+    context (repeated calls reuse its index buffers)."""
     gold = json.load(open(os.path.join(GOLDEN, "skewed_60m.json")))
     if "msp_prefix_sha256" not in gold:
         pytest.skip("the oracle's prefix state of skewed_60m is not recorded (make_golden.py --msp-prefix)")
     g = gb.Graph.generate("skewed_60m")
+    # the per-pivot oracle's 800-level state, then the bloom oracle's deeper
+    # checkpoints (its 800-level state is the per-pivot one)
+    want = {gold["msp_prefix_levels"]: gold["msp_prefix_sha256"]}
+    want.update({int(k): v for k, v in gold.get("msp_prefix_checkpoints", {}).items()})
     with gb.Device(dev.device) as d:
-        d.set_peel_max_level(gold["msp_prefix_levels"])
+        # the bloom-index peel or a failure: never the merge path, which
+        # would take hours here
+        d.set_msp_path("index")
         dg = d.upload(g)
-        say(f"skewed_60m msp through level {gold['msp_prefix_levels']}")
-        res, _ = dg.msp()
-        assert dg.partial
+        for level in sorted(want):
+            d.set_peel_max_level(level)
+            say(f"skewed_60m msp through level {level}")
+            res, _ = dg.msp()
+            assert dg.partial
+            assert digest(res) == want[level], level
         dg.free()
-    assert digest(res) == gold["msp_prefix_sha256"]
