@@ -152,3 +152,27 @@ def test_batch_msp_paths_agree(dev):
     b.free()
     for a, c in zip(merge, index):
         assert np.array_equal(a, c)
+
+
+@pytest.mark.gpu
+def test_batch_full_mix_equals_plain(dev):
+    """The batched configuration at full size: all 4,096 graphs of the mix
+    (seeds 1000..5095, 86.7 M edges) decomposed as batches, every graph's
+    msp results and level count against the plain call on that graph."""
+    k = 4096
+    gs = [gb.Graph.generate("mix_graph", i) for i in range(k)]
+    b = dev.upload_batch(gs)
+    try:
+        say(f"full mix: {sum(g.m for g in gs)} edges in {b.chunks()} unions")
+        arrays, scal = b.run("msp")
+    finally:
+        b.free()
+    same = 0
+    for i, g in enumerate(gs):
+        dg = dev.upload(g)
+        r, lv = dg.msp()
+        dg.free()
+        same += int(np.array_equal(r, arrays[i]) and lv == scal[i]["levels"])
+        if i % 1024 == 1023:
+            say(f"compared {i + 1} graphs, {same} identical")
+    assert same == k

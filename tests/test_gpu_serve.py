@@ -109,3 +109,54 @@ def test_serve_budget(dev):
         assert srv.stats()["peak_bytes"] <= 1 << 30
     for i, (arr, scal, _) in enumerate(outs):
         check(dev, gb.Graph.generate("mix_graph", i), "msp", arr, scal)
+
+
+@pytest.mark.gpu
+def test_serve_poisson_stream_as_configured(dev):
+    """The serving configuration: a Poisson stream (seed 7, 50 requests/s)
+    of mix graphs with 2% large Chung-Lu requests (m in [1M, 5M]), here its
+    first 40 requests, which hold three large ones (the first 1.56M edges).
+    Every result equals the plain call on the same graph, and the first
+    large request's equals the oracle's digest."""
+    import hashlib
+    import json
+    import os
+
+    rng = np.random.default_rng(7)
+    n_all = 400
+    large = rng.random(n_all) < 0.02
+    idx = rng.integers(0, 4096, n_all)
+    n = 40
+    reqs = [gb.Graph.generate("large_request", int(i)) if big else gb.Graph.generate("mix_graph", int(i))
+            for i, big in zip(idx[:n], large[:n])]
+    assert int(large[:n].sum()) == 3 and all(reqs[p].m >= 1_000_000 for p in np.nonzero(large[:n])[0])
+    gaps = rng.exponential(1.0 / 50.0, n)
+    with gb.Server(large_edges=1_000_000) as srv:
+        tickets = [None] * n
+
+        def feed():
+            t = time.perf_counter()
+            for i, g in enumerate(reqs):
+                t += gaps[i]
+                d = t - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tickets[i] = srv.submit(g, "msp")
+
+        th = threading.Thread(target=feed)
+        th.start()
+        th.join()
+        results = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    lat = np.array([r[2] for r in results])
+    say(f"stream of {n}: p50 {np.percentile(lat, 50):.1f} ms, p99 {np.percentile(lat, 99):.1f} ms, {st}")
+    assert st["completed"] == n and st["failed"] == 0 and st["large_runs"] >= 1
+    gold = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                       "large_request_1204.json")))
+    first = int(np.nonzero(large[:n])[0][0])
+    assert int(idx[first]) == gold["index"] and reqs[first].m == gold["m"]
+    arr = results[first][0]
+    assert hashlib.sha256(np.ascontiguousarray(arr, dtype="<u8").tobytes()).hexdigest() == gold["msp_sha256"]
+    assert results[first][1]["levels"] == gold["msp_levels"]
+    for (arr, scal, _), g in zip(results, reqs):
+        check(dev, g, "msp", arr, scal)
