@@ -17,10 +17,14 @@ from .device import (
     default_device,
     device_count,
     g_bitruss_hindex,
+    tune_candidates,
+    tune_class,
+    tune_default,
 )
 from ._lib import BitrussError, LIB_PATH, COH_PATH
 
 __all__ = [
     "Graph", "WORKLOADS", "Batch", "Device", "Server", "DeviceGraph", "bfc_vpp", "bfc_evpp", "g_bitruss_hindex",
     "bitruss_msp", "bitruss_mspp", "bitruss_peeling", "default_device", "device_count", "BitrussError", "LIB_PATH", "COH_PATH",
+    "tune_candidates", "tune_class", "tune_default",
 ]

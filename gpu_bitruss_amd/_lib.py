@@ -97,6 +97,19 @@ class BtServeStats(C.Structure):
         "in_use_bytes")]
 
 
+class BtCountConfig(C.Structure):
+    """struct bt_count_config (include/bitruss_hip.h)."""
+
+    _fields_ = [("wave_wedges", C.c_uint32), ("block_wedges", C.c_uint32), ("block_threads", C.c_uint32),
+                ("hub_threads", C.c_uint32)]
+
+    def as_tuple(self) -> tuple:
+        return (self.wave_wedges, self.block_wedges, self.block_threads, self.hub_threads)
+
+
+TUNE_CLASSES = 16
+
+
 class BitrussError(RuntimeError):
     """A non-zero status returned by libbitruss_hip.so."""
 
@@ -117,6 +130,7 @@ OPT_PEEL_MAX_LEVEL = 8
 OPT_PEEL_C2CAP = 9
 OPT_PEEL_WINDOW_EDGES = 10
 OPT_PEEL_BIG_MIDDLES = 11
+OPT_HIDX_GRAPH_ROUNDS = 12
 # statuses (enum bt_status, include/bt_graph.h) the wrappers act on
 BT_PARTIAL = 10
 BT_ERR_BUSY = 11
@@ -156,6 +170,9 @@ def lib() -> C.CDLL:
         "bt_gen_workload": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(BtEdgeList)]),
         "bt_graph_wedges": (C.c_int, [C.POINTER(BtHostGraph), C.POINTER(C.c_uint64)]),
         "bt_graph_union": (C.c_int, [C.POINTER(BtHostGraph), C.c_uint64, C.POINTER(BtHostGraph), u64p]),
+        "bt_tune_class": (C.c_int, [C.c_uint64, C.c_uint32]),
+        "bt_tune_default": (C.c_int, [C.c_int, C.POINTER(BtCountConfig)]),
+        "bt_tune_candidates": (C.c_int, [C.POINTER(BtCountConfig), C.c_int, C.POINTER(C.c_int)]),
     }
     gpu_sig = {
         "bt_device_count": (C.c_int, [C.POINTER(C.c_int)]),
@@ -164,6 +181,10 @@ def lib() -> C.CDLL:
         "bt_ctx_last_timings": (C.c_int, [vp, C.POINTER(BtTimings)]),
         "bt_ctx_set_option": (C.c_int, [vp, C.c_int, C.c_int64]),
         "bt_ctx_trim": (C.c_int, [vp]),
+        "bt_tune_get": (C.c_int, [vp, C.c_int, C.POINTER(BtCountConfig)]),
+        "bt_tune_set": (C.c_int, [vp, C.c_int, C.POINTER(BtCountConfig)]),
+        "bt_autotune": (C.c_int, [vp, vp, C.c_int, C.POINTER(BtCountConfig), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]),
         "bt_graph_upload": (C.c_int, [vp, C.POINTER(BtHostGraph), C.POINTER(vp)]),
         "bt_graph_free": (C.c_int, [vp]),
         "bt_bfc": (C.c_int, [vp, vp, u64p]),

@@ -327,6 +327,8 @@ int bt_batch_run(bt_ctx* ctx, bt_batch* b, int algo, uint64_t* const* outs, bt_b
         lc->peel_c2cap = ctx->peel_c2cap;
         lc->peel_window_edges = ctx->peel_window_edges;
         lc->peel_big_middles = ctx->peel_big_middles;
+        lc->hidx_graph_rounds = ctx->hidx_graph_rounds;
+        std::copy(ctx->tune, ctx->tune + BT_TUNE_CLASSES, lc->tune);
         lc->peel_blocks = share < lc->cu_count ? share : 0;
     }
     std::vector<int> status(nl, BT_OK);

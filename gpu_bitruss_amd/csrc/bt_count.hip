@@ -11,13 +11,18 @@
 // priority rank once per call, each row sorted by rank, so for root r the
 // lower-priority neighbours are a row prefix (r_lo[r] entries) and for each
 // such v the valid far ends are the prefix of row v below r (r_p entries).
-// Roots are bucketed by their wedge count W_r:
-//   A  W_r <= 1024 : one 64-lane wavefront per root, packed LDS hash (8 KB)
-//   B  W_r <= 8192 : one 256-thread workgroup per root, LDS hash (64 KB)
-//   C  the rest    : hub roots; chunks of their rows are spread over the
-//                    whole grid and count into dense per-root tables in HBM
+// Roots are bucketed by their wedge count W_r (bounds and workgroup sizes
+// per graph-size class from the context's tuning table, bt_tune.hip):
+//   A  W_r <= wave_wedges  (256..2048): one 64-lane wavefront per root,
+//                          packed LDS hash of 2 * wave_wedges slots
+//   B  W_r <= block_wedges (4096..16384): one workgroup of block_threads
+//                          per root, LDS hash of 2 * block_wedges slots
+//   C  the rest: hub roots; chunks of hub_threads row entries are spread
+//                over the whole grid and count into dense per-root tables
+//                in HBM
 // An LDS hash slot packs (w+1) in the high bits and the count in the low
-// `cb` bits; roots with r_lo >= 2^cb go to bucket C.
+// `cb` bits; roots with r_lo >= 2^cb go to bucket C. A hash never holds
+// more keys than half its slots (distinct far ends <= wedges of the root).
 #include <hip/hip_runtime.h>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -35,12 +40,6 @@ using namespace bt;
 
 namespace {
 
-constexpr uint32_t kWA = 1024;       // bucket A wedge limit
-constexpr uint32_t kCapA = 2048;     // bucket A hash slots
-constexpr uint32_t kWB = 8192;       // bucket B wedge limit
-constexpr uint32_t kCapB = 16384;    // bucket B hash slots
-constexpr int kBlockB = 256;
-constexpr int kBlockC = 256;
 
 // ------------------------------------------------------------------ prep ---
 
@@ -123,16 +122,16 @@ __global__ void k_root_w(uint32_t n, const uint32_t* __restrict__ r_off, const u
 }
 
 __global__ void k_classify(uint32_t n, const uint64_t* __restrict__ r_w, const uint32_t* __restrict__ r_lo,
-                           uint32_t lo_limit, uint32_t* __restrict__ la, uint32_t* __restrict__ lb,
-                           uint32_t* __restrict__ lc, uint64_t* __restrict__ ctl) {
+                           uint32_t lo_limit, uint32_t wa, uint32_t wb, uint32_t* __restrict__ la,
+                           uint32_t* __restrict__ lb, uint32_t* __restrict__ lc, uint64_t* __restrict__ ctl) {
     uint32_t const r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     uint64_t const w = r_w[r];
     if (w == 0) return;
     uint32_t const lo = r_lo[r];
-    if (lo < lo_limit && w <= kWA) {
+    if (lo < lo_limit && w <= wa) {
         la[atomicAdd(reinterpret_cast<unsigned long long*>(&ctl[0]), 1ull)] = r;
-    } else if (lo < lo_limit && w <= kWB) {
+    } else if (lo < lo_limit && w <= wb) {
         lb[atomicAdd(reinterpret_cast<unsigned long long*>(&ctl[1]), 1ull)] = r;
     } else {
         lc[atomicAdd(reinterpret_cast<unsigned long long*>(&ctl[2]), 1ull)] = r;
@@ -344,6 +343,63 @@ int bits_for(uint64_t x) {
     return b;
 }
 
+// Launch of buckets A and B for the tuned settings: the LDS table has twice
+// the bucket's wedge bound in slots.
+struct LdsLaunch {
+    const uint32_t* roots;
+    uint32_t nroots;
+    RowView g;
+    const uint32_t* r_lo;
+    uint64_t* sup;
+    uint64_t* total;
+    int cb, per_edge;
+    Emit em;
+    unsigned grid_cap;
+    hipStream_t s;
+    template <int BLOCK, uint32_t CAP>
+    void run() const {
+        k_count_lds<BLOCK, CAP><<<grid_for(nroots, 1, grid_cap), BLOCK, 0, s>>>(roots, nroots, g, r_lo, sup, total,
+                                                                               cb, per_edge, em);
+    }
+    int bucket_a(uint32_t wave_wedges) const {
+        switch (wave_wedges) {
+            case 256: run<kWave, 512>(); break;
+            case 512: run<kWave, 1024>(); break;
+            case 1024: run<kWave, 2048>(); break;
+            case 2048: run<kWave, 4096>(); break;
+            default: return BT_ERR_INVALID;
+        }
+        return BT_OK;
+    }
+    int bucket_b(uint32_t block_wedges, uint32_t threads) const {
+        if (threads == 256) {
+            switch (block_wedges) {
+                case 4096: run<256, 8192>(); break;
+                case 8192: run<256, 16384>(); break;
+                case 16384: run<256, 32768>(); break;
+                default: return BT_ERR_INVALID;
+            }
+        } else if (threads == 512) {
+            switch (block_wedges) {
+                case 4096: run<512, 8192>(); break;
+                case 8192: run<512, 16384>(); break;
+                case 16384: run<512, 32768>(); break;
+                default: return BT_ERR_INVALID;
+            }
+        } else {
+            return BT_ERR_INVALID;
+        }
+        return BT_OK;
+    }
+};
+
+template <int BLOCK>
+void launch_dense(unsigned grid, hipStream_t s, const HubItem* items, uint32_t nitems, const RowView& g,
+                  uint32_t* pool, uint64_t* sup, uint64_t* total, const Emit& em, bool per_edge) {
+    k_count_dense<BLOCK, 1><<<grid, BLOCK, 0, s>>>(items, nitems, g, pool, sup, total, em);
+    if (per_edge) k_count_dense<BLOCK, 2><<<grid, BLOCK, 0, s>>>(items, nitems, g, pool, sup, total, em);
+}
+
 }  // namespace
 
 int bt_prep_priority(bt_dgraph* dg, const bt_fixed_scratch* fixed) {
@@ -420,8 +476,10 @@ int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total_out, uint64_t* s
     uint32_t* la = dg->list0;
     uint32_t* lb = dg->list1;
     uint32_t* lc = dg->r_val2;
-    k_classify<<<grid_for(n, T, 1u << 30), T, 0, s>>>(n, dg->r_w, dg->r_lo, cb > 0 ? lo_limit : 0, la, lb, lc,
-                                                      dg->ctl);
+    bt_count_config const cfg = bt_count_settings(dg);
+    if (!bt_count_config_valid(cfg)) return BT_ERR_INVALID;
+    k_classify<<<grid_for(n, T, 1u << 30), T, 0, s>>>(n, dg->r_w, dg->r_lo, cb > 0 ? lo_limit : 0, cfg.wave_wedges,
+                                                      cfg.block_wedges, la, lb, lc, dg->ctl);
     uint64_t cnts[4];
     BT_TRY(bt_read_ctl(dg, 0, 4, cnts));
     uint64_t const na = cnts[0], nb = cnts[1], nc = cnts[2];
@@ -461,11 +519,11 @@ int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total_out, uint64_t* s
     unsigned const max_grid = unsigned(c->cu_count) * 16;
     BT_TRY(tm.start(2));
     if (na)
-        k_count_lds<kWave, kCapA><<<grid_for(na, 1, max_grid * 4), kWave, 0, s>>>(la, uint32_t(na), g, dg->r_lo,
-                                                                               dg->sup, total, cb, per_edge, em);
+        BT_TRY((LdsLaunch{la, uint32_t(na), g, dg->r_lo, dg->sup, total, cb, per_edge, em, max_grid * 4, s}
+                    .bucket_a(cfg.wave_wedges)));
     if (nb)
-        k_count_lds<kBlockB, kCapB><<<grid_for(nb, 1, max_grid), kBlockB, 0, s>>>(lb, uint32_t(nb), g, dg->r_lo,
-                                                                               dg->sup, total, cb, per_edge, em);
+        BT_TRY((LdsLaunch{lb, uint32_t(nb), g, dg->r_lo, dg->sup, total, cb, per_edge, em, max_grid, s}
+                    .bucket_b(cfg.block_wedges, cfg.block_threads)));
     BT_HIP_TRY(hipGetLastError());
     if (nc) {
         // hub roots: build (root, entry chunk) items on the host, batch them
@@ -503,8 +561,8 @@ int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total_out, uint64_t* s
             while (k2 < nc && used + hroots[k2] <= pool_entries) {
                 uint32_t const r = hroots[k2];
                 uint32_t const b = all_off[r], lo = all_lo[r];
-                for (uint32_t i = 0; i < lo; i += kBlockC)
-                    items.push_back(HubItem{r, b + i, b + std::min<uint32_t>(lo, i + kBlockC), 0, used});
+                for (uint32_t i = 0; i < lo; i += cfg.hub_threads)
+                    items.push_back(HubItem{r, b + i, b + std::min<uint32_t>(lo, i + cfg.hub_threads), 0, used});
                 used += r;
                 k2++;
             }
@@ -512,11 +570,10 @@ int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total_out, uint64_t* s
             BT_HIP_TRY(hipMemsetAsync(pool, 0, used * sizeof(uint32_t), s));
             BT_HIP_TRY(hipMemcpyAsync(ditems, items.data(), items.size() * sizeof(HubItem), hipMemcpyHostToDevice, s));
             unsigned const gC = grid_for(items.size(), 1, max_grid);
-            k_count_dense<kBlockC, 1><<<gC, kBlockC, 0, s>>>(ditems, uint32_t(items.size()), g, pool, dg->sup, total,
-                                                             em);
-            if (per_edge)
-                k_count_dense<kBlockC, 2><<<gC, kBlockC, 0, s>>>(ditems, uint32_t(items.size()), g, pool, dg->sup,
-                                                                 total, em);
+            if (cfg.hub_threads == 512)
+                launch_dense<512>(gC, s, ditems, uint32_t(items.size()), g, pool, dg->sup, total, em, per_edge);
+            else
+                launch_dense<256>(gC, s, ditems, uint32_t(items.size()), g, pool, dg->sup, total, em, per_edge);
             BT_HIP_TRY(hipGetLastError());
             // the host vector `items` is reused only after this batch's copy
             BT_HIP_TRY(hipStreamSynchronize(s));

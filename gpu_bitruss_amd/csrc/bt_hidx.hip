@@ -22,6 +22,16 @@
 // the values are streamed again inside that bin (two passes up to
 // prev[e] ~ 10^6, a third above). One wave per edge; the wave walks the
 // concatenation of the edge's blooms 64 middles at a time.
+//
+// Round loop: the reference reads a change flag back after every round
+// (bitruss_hidx.cu:219-231). Here the rounds are captured once into a HIP
+// graph of K rounds (BT_OPT_HIDX_GRAPH_ROUNDS, default 8) whose kernels keep
+// the loop state on the device: a round counter, the change flag and a done
+// flag that the first round without a change sets, after which the graph's
+// remaining rounds return at once. The host launches the graph and reads
+// the state back once per K rounds. The buffers alternate by the position
+// of a round in the graph (K is even), and the round without a change
+// leaves both buffers equal, so the result is in either.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -41,8 +51,46 @@ constexpr int kHT = 512;              // block size
 constexpr int kHW = kHT / kWave;      // waves per block
 constexpr uint32_t kHB = 1024;        // histogram bins per wave
 
+// Device-side loop state (control words of the graph): round counter,
+// change flag of the current round, done flag.
+struct LoopState {
+    unsigned long long* round;
+    unsigned long long* changed;
+    unsigned long long* done;
+};
+
+__global__ void k_hidx_loop_init(LoopState st) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        *st.round = 0;
+        *st.changed = 0;
+        *st.done = 0;
+    }
+}
+
+// Start of a round: count it and clear its change flag (unless done).
+__global__ void k_hidx_round_begin(LoopState st) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !*st.done) {
+        *st.round += 1;
+        *st.changed = 0;
+    }
+}
+
+// End of a round: a round without a change ends the loop.
+__global__ void k_hidx_round_end(LoopState st) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !*st.done && !*st.changed) *st.done = 1;
+}
+
+// Clears a round's bloom marks (a kernel, not a memset node: the graph
+// replays it).
+__global__ void k_hidx_clear(uint8_t* __restrict__ p, size_t n, const unsigned long long* done) {
+    if (done && *done) return;
+    for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        p[i] = 0;
+}
+
 __global__ void k_hidx_mv(uint32_t R, const uint64_t* __restrict__ rec, const uint64_t* __restrict__ prev,
-                          uint64_t* __restrict__ mv) {
+                          uint64_t* __restrict__ mv, const unsigned long long* done) {
+    if (done && *done) return;
     for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < R; p += uint64_t(gridDim.x) * blockDim.x) {
         uint64_t const r = rec[p];
         uint64_t const a = prev[uint32_t(r >> 32)], b = prev[uint32_t(r)];
@@ -68,6 +116,10 @@ struct HidxArgs {
     // (null: every bloom, the first round), and the marks of this round
     const uint8_t* dirty_in;
     uint8_t* dirty_out;
+    // graph rounds: the device loop state (null: round and dirty_in above
+    // hold); the round number is read from it, and round 1 ignores dirty_in
+    const unsigned long long* dev_round;
+    const unsigned long long* done;
 };
 
 // Whether any bloom holding edge e (records q0..q1 of its incidence list)
@@ -139,6 +191,11 @@ __device__ __forceinline__ void bin_values(const HidxArgs& a, uint32_t e, uint64
 }
 
 __global__ __launch_bounds__(kHT) void k_hidx_round(HidxArgs a) {
+    if (a.done) {
+        if (*a.done) return;
+        a.round = uint32_t(*a.dev_round);
+        if (a.round == 1) a.dirty_in = nullptr;
+    }
     __shared__ uint32_t s_hist[kHW][kHB];
     __shared__ uint32_t s_incl[kHW][kWave], s_base[kHW][kWave], s_skip[kHW][kWave];
     __shared__ uint64_t s_pp[kHW][kWave];
@@ -227,6 +284,86 @@ __global__ __launch_bounds__(kHT) void k_hidx_round(HidxArgs a) {
 
 }  // namespace
 
+// The rounds as a captured graph of K rounds (K even), launched until a
+// round without a change; *rounds receives the rounds counted on the device.
+static int hidx_graph_loop(bt_dgraph* dg, uint64_t* mv, unsigned long long* changed, uint8_t* const dirty[2],
+                           size_t nbm, unsigned grid_mv, unsigned grid_e, uint32_t* rounds) {
+    bt_ctx* c = dg->ctx;
+    hipStream_t s = c->stream;
+    bt_index& ix = c->ix;
+    uint32_t const R = uint32_t(ix.R), m = dg->m;
+    int const K = c->hidx_graph_rounds + (c->hidx_graph_rounds & 1);
+    LoopState const st{reinterpret_cast<unsigned long long*>(dg->ctl + 17), changed,
+                       reinterpret_cast<unsigned long long*>(dg->ctl + 18)};
+    k_hidx_loop_init<<<1, 64, 0, s>>>(st);
+    BT_HIP_TRY(hipGetLastError());
+    unsigned const grid_clear = grid_for(nbm, 256, unsigned(c->cu_count) * 8);
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    BT_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < K; k++) {
+        // position k: round parity k & 1 (rounds start at 1 on position 0)
+        uint64_t* prev = (k & 1) ? dg->aux : dg->res;
+        uint64_t* curr = (k & 1) ? dg->res : dg->aux;
+        uint8_t* out = dirty[(k + 1) & 1];  // round r writes dirty[r & 1]
+        uint8_t* in = dirty[k & 1];         // and reads dirty[(r + 1) & 1]
+        k_hidx_round_begin<<<1, 64, 0, s>>>(st);
+        k_hidx_clear<<<grid_clear, 256, 0, s>>>(out, nbm, st.done);
+        if (R) k_hidx_mv<<<grid_mv, 256, 0, s>>>(R, static_cast<const uint64_t*>(ix.vals1.ptr), prev, mv, st.done);
+        HidxArgs a{m,
+                   prev,
+                   curr,
+                   static_cast<const uint64_t*>(ix.inc_off.ptr),
+                   static_cast<const uint32_t*>(ix.inc.ptr),
+                   static_cast<const uint64_t*>(ix.vals1.ptr),
+                   static_cast<const uint32_t*>(ix.bid.ptr),
+                   static_cast<const uint32_t*>(ix.bstart.ptr),
+                   mv,
+                   changed,
+                   dg->egraph,
+                   dg->graph_last,
+                   0,
+                   in,
+                   out,
+                   st.round,
+                   st.done};
+        k_hidx_round<<<grid_e, kHT, 0, s>>>(a);
+        k_hidx_round_end<<<1, 64, 0, s>>>(st);
+    }
+    hipError_t const ce = hipStreamEndCapture(s, &graph);
+    if (ce != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        BT_HIP_TRY(ce);
+    }
+    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    BT_HIP_TRY(ie);
+    uint64_t* h = static_cast<uint64_t*>(c->h_pinned);
+    bool const progress = getenv("BT_PEEL_PROGRESS") != nullptr;
+    auto const t_first = std::chrono::steady_clock::now();
+    int st_out = BT_OK;
+    for (;;) {
+        hipError_t e = hipGraphLaunch(exec, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(h, dg->ctl + 17, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            bt_set_hip_error(e, "hidx graph rounds", __FILE__, __LINE__);
+            st_out = BT_ERR_DEVICE;
+            break;
+        }
+        if (progress) {
+            double const sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_first).count();
+            std::fprintf(stderr, "[bt_hidx] %llu rounds%s, %.1f s\n", (unsigned long long)h[0],
+                         h[1] ? " (done)" : "", sec);
+            std::fflush(stderr);
+        }
+        if (h[1]) break;
+    }
+    (void)hipGraphExecDestroy(exec);
+    if (st_out == BT_OK) *rounds = uint32_t(h[0]);
+    return st_out;
+}
+
 int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds_out) {
     bt_ctx* c = dg->ctx;
     hipStream_t s = c->stream;
@@ -256,11 +393,15 @@ int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds_out) {
         uint8_t* dirty[2] = {static_cast<uint8_t*>(ix.hdirty.ptr), static_cast<uint8_t*>(ix.hdirty.ptr) + nbm};
         bool const progress = getenv("BT_PEEL_PROGRESS") != nullptr;
         auto const t_first = std::chrono::steady_clock::now();
-        for (;;) {
+        if (c->hidx_graph_rounds > 0) {
+            BT_TRY(hidx_graph_loop(dg, mv, changed, dirty, nbm, grid_mv, grid_e, &rounds));
+            prev = dg->res;  // the last round left both buffers equal
+        }
+        while (c->hidx_graph_rounds == 0) {
             rounds++;
             BT_HIP_TRY(hipMemsetAsync(changed, 0, sizeof(uint64_t), s));
             BT_HIP_TRY(hipMemsetAsync(dirty[rounds & 1], 0, nbm, s));
-            if (R) k_hidx_mv<<<grid_mv, 256, 0, s>>>(R, static_cast<const uint64_t*>(ix.vals1.ptr), prev, mv);
+            if (R) k_hidx_mv<<<grid_mv, 256, 0, s>>>(R, static_cast<const uint64_t*>(ix.vals1.ptr), prev, mv, nullptr);
             HidxArgs a{m,
                        prev,
                        curr,
@@ -275,7 +416,9 @@ int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds_out) {
                        dg->graph_last,
                        rounds,
                        rounds > 1 ? dirty[(rounds + 1) & 1] : nullptr,
-                       dirty[rounds & 1]};
+                       dirty[rounds & 1],
+                       nullptr,
+                       nullptr};
             k_hidx_round<<<grid_e, kHT, 0, s>>>(a);
             BT_HIP_TRY(hipGetLastError());
             BT_HIP_TRY(hipMemcpyAsync(h_changed, changed, sizeof(uint64_t), hipMemcpyDeviceToHost, s));

@@ -111,6 +111,12 @@ struct bt_ctx {
     int peel_c2cap = 128;          // both-frontier middles summed from the on-chip list
     uint64_t peel_window_edges = uint64_t(4) << 20;  // live edges from which the collect uses a window
     int peel_big_middles = 1024;   // blooms beyond this are walked by a whole workgroup
+    int hidx_graph_rounds = 8;     // h-index rounds per captured graph (0: one by one)
+    // counting launch settings per graph-size class (bt_tune_*), and the
+    // settings an autotuning run imposes on the calls it times (null: the
+    // table decides)
+    bt_count_config tune[BT_TUNE_CLASSES]{};
+    const bt_count_config* count_force = nullptr;
     // device blocks given back by freed graphs and batches, reused by the
     // next ones (hipFree would wait for every stream on the device)
     std::mutex cache_mu;
@@ -218,3 +224,9 @@ private:
 
 // Read back a few control words (synchronises the stream).
 int bt_read_ctl(bt_dgraph* dg, int first, int count, uint64_t* out);
+
+// The counting settings a call on dg uses: the forced ones of an
+// autotuning run, else the context's entry for dg's size class.
+bt_count_config bt_count_settings(const bt_dgraph* dg);
+// Whether cfg is one of the settings the counting kernels are built for.
+bool bt_count_config_valid(const bt_count_config& cfg);

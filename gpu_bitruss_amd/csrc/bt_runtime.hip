@@ -227,6 +227,7 @@ int bt_ctx_create(int device, bt_ctx** out) {
         return BT_ERR_DEVICE;
     }
     c->cu_count = prop.multiProcessorCount;
+    for (int k = 0; k < BT_TUNE_CLASSES; k++) bt_tune_default(k, &c->tune[k]);
     c->coop = prop.cooperativeLaunch;
     c->lds_per_block = prop.sharedMemPerBlock;
     for (auto& e : c->ev) {
@@ -311,6 +312,10 @@ int bt_ctx_set_option(bt_ctx* ctx, int option, int64_t value) {
         case BT_OPT_PEEL_WINDOW_EDGES:
             if (value < 0) return BT_ERR_INVALID;
             ctx->peel_window_edges = uint64_t(value);
+            return BT_OK;
+        case BT_OPT_HIDX_GRAPH_ROUNDS:
+            if (value < 0 || value > 1024) return BT_ERR_INVALID;
+            ctx->hidx_graph_rounds = int(value);
             return BT_OK;
         case BT_OPT_PEEL_BIG_MIDDLES:
             if (value < 1 || value > 0x7FFFFFFF) return BT_ERR_INVALID;

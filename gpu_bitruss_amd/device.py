@@ -33,6 +33,26 @@ def _release_all() -> None:
 atexit.register(_release_all)
 
 
+def tune_class(edges: int, graphs: int = 1) -> int:
+    """Graph-size class of the counting settings: floor(log4(edges per graph))."""
+    return _lib.lib().bt_tune_class(edges, graphs)
+
+
+def tune_default(size_class: int) -> tuple:
+    """The counting settings tuned on MI355X for a size class."""
+    c = _lib.BtCountConfig()
+    _lib.check("bt_tune_default", _lib.lib().bt_tune_default(size_class, C.byref(c)))
+    return c.as_tuple()
+
+
+def tune_candidates() -> list:
+    n = C.c_int(0)
+    _lib.check("bt_tune_candidates", _lib.lib().bt_tune_candidates(None, 0, C.byref(n)))
+    cands = (_lib.BtCountConfig * n.value)()
+    _lib.check("bt_tune_candidates", _lib.lib().bt_tune_candidates(cands, n.value, C.byref(n)))
+    return [c.as_tuple() for c in cands]
+
+
 def device_count() -> int:
     c = C.c_int(0)
     _lib.check("bt_device_count", _lib.lib().bt_device_count(C.byref(c)))
@@ -99,6 +119,36 @@ class Device:
     def set_peel_big_middles(self, middles: int) -> None:
         """Blooms with more middles are walked by a whole workgroup."""
         _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(self.handle, _lib.OPT_PEEL_BIG_MIDDLES, middles))
+
+    def set_hidx_graph_rounds(self, rounds: int) -> None:
+        """h-index rounds per captured HIP graph (0: one by one, with a
+        read-back after each round)."""
+        _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(self.handle, _lib.OPT_HIDX_GRAPH_ROUNDS, rounds))
+
+    def tune_get(self, size_class: int) -> tuple:
+        """Counting settings of a graph-size class: (wave_wedges,
+        block_wedges, block_threads, hub_threads)."""
+        c = _lib.BtCountConfig()
+        _lib.check("bt_tune_get", _lib.lib().bt_tune_get(self.handle, size_class, C.byref(c)))
+        return c.as_tuple()
+
+    def tune_set(self, size_class: int, cfg) -> None:
+        c = _lib.BtCountConfig(*cfg)
+        _lib.check("bt_tune_set", _lib.lib().bt_tune_set(self.handle, size_class, C.byref(c)))
+
+    def autotune(self, dgraph: "DeviceGraph", reps: int = 3):
+        """Times every counting candidate on a resident graph, keeps the
+        fastest for its size class; returns (best settings, best ms,
+        {settings: ms})."""
+        n = C.c_int(0)
+        _lib.check("bt_tune_candidates", _lib.lib().bt_tune_candidates(None, 0, C.byref(n)))
+        cands = (_lib.BtCountConfig * n.value)()
+        _lib.check("bt_tune_candidates", _lib.lib().bt_tune_candidates(cands, n.value, C.byref(n)))
+        ms = (C.c_double * n.value)()
+        best, best_ms = _lib.BtCountConfig(), C.c_double()
+        _lib.check("bt_autotune", _lib.lib().bt_autotune(self.handle, dgraph.handle, reps, C.byref(best),
+                                                          C.byref(best_ms), ms))
+        return best.as_tuple(), best_ms.value, {c.as_tuple(): ms[i] for i, c in enumerate(cands)}
 
     def trim(self) -> None:
         """Give back the device memory the context caches between calls."""

@@ -131,11 +131,18 @@ int bt_ctx_trim(bt_ctx* ctx);
  *   BT_OPT_PEEL_WINDOW_EDGES  live edges from which the level collect scans a
  *                          support window (default 4,194,304; 0 uses it on
  *                          every graph). Results are the same.
+ *   BT_OPT_HIDX_GRAPH_ROUNDS  h-index rounds per captured HIP graph (default 8;
+ *                          odd values round up): the round loop keeps its
+ *                          state on the device and the host reads it back
+ *                          once per graph launch; 0 runs the rounds one by
+ *                          one with a read-back after each, as the reference
+ *                          (bitruss_hidx.cu:219-231)
  *   BT_OPT_PEEL_BIG_MIDDLES  blooms with more middles are walked by a whole
  *                          workgroup (default 1,024). Results are the same. */
 enum { BT_OPT_MSP_PATH = 1, BT_OPT_INDEX_MEM_PCT = 2, BT_OPT_BATCH_WEDGES = 3, BT_OPT_BATCH_LEVELS = 4,
        BT_OPT_PEEL_BLOCKS = 5, BT_OPT_PEEL_SLICE_MS = 6, BT_OPT_BATCH_LANES = 7, BT_OPT_PEEL_MAX_LEVEL = 8,
-       BT_OPT_PEEL_C2CAP = 9, BT_OPT_PEEL_WINDOW_EDGES = 10, BT_OPT_PEEL_BIG_MIDDLES = 11 };
+       BT_OPT_PEEL_C2CAP = 9, BT_OPT_PEEL_WINDOW_EDGES = 10, BT_OPT_PEEL_BIG_MIDDLES = 11,
+       BT_OPT_HIDX_GRAPH_ROUNDS = 12 };
 enum { BT_MSP_PATH_AUTO = 0, BT_MSP_PATH_MERGE = 1, BT_MSP_PATH_INDEX = 2 };
 enum { BT_BATCH_LEVELS_PER_GRAPH = 0, BT_BATCH_LEVELS_UNION = 1 };
 int bt_ctx_set_option(bt_ctx* ctx, int option, int64_t value);
@@ -170,6 +177,41 @@ int bt_ebfc(bt_ctx* ctx, bt_dgraph* dg, uint64_t* support_out, uint64_t* total,
 #define BT_STATIC_SCRATCH_BYTES (128u << 20)
 int bt_ebfc_static(bt_ctx* ctx, bt_dgraph* dg, uint64_t* support_out, uint64_t* total,
                    uint64_t* support_max);
+
+/* ---- counting launch settings per graph-size class -----------------------
+ *
+ * Replace the compile-time launch constants of the reference's counting
+ * kernels (BLK_NUMS / BLK_DIM, src/util/config.h:9-10). Roots are split by
+ * their priority-wedge count: up to wave_wedges, one 64-lane wavefront per
+ * root with an LDS hash table of 2 * wave_wedges slots; up to block_wedges,
+ * one workgroup of block_threads per root with an LDS table of
+ * 2 * block_wedges slots; above (the hubs), chunks of hub_threads row
+ * entries spread over the device, counting into dense tables in HBM. The
+ * settings change only the speed, never a result. A graph's size class is
+ * floor(log4(edges per graph)) (a batch union: its edges over its graphs),
+ * clamped to BT_TUNE_CLASSES - 1. A context starts from the table tuned on
+ * MI355X (bt_tune_default); bt_autotune measures every candidate on a
+ * graph and keeps the fastest for that graph's class. */
+#define BT_TUNE_CLASSES 16
+typedef struct bt_count_config {
+    uint32_t wave_wedges;   /* 256, 512, 1024 or 2048 */
+    uint32_t block_wedges;  /* 4096, 8192 or 16384 */
+    uint32_t block_threads; /* 256 or 512 */
+    uint32_t hub_threads;   /* 256 or 512 */
+} bt_count_config;
+int bt_tune_class(uint64_t edges, uint32_t graphs);
+int bt_tune_default(int size_class, bt_count_config* out);
+/* the candidates bt_autotune measures: *n receives their number; out (may
+ * be NULL) receives them, at most cap entries */
+int bt_tune_candidates(bt_count_config* out, int cap, int* n);
+int bt_tune_get(const bt_ctx* ctx, int size_class, bt_count_config* out);
+int bt_tune_set(bt_ctx* ctx, int size_class, const bt_count_config* cfg);
+/* Times the per-edge support count of dg under every candidate (best of
+ * reps runs each), checks that every candidate's supports equal the first
+ * one's (BT_ERR_INTERNAL otherwise), and sets the fastest for dg's class.
+ * best and best_ms may be NULL; ms (may be NULL) receives the time of each
+ * candidate in bt_tune_candidates order. */
+int bt_autotune(bt_ctx* ctx, bt_dgraph* dg, int reps, bt_count_config* best, double* best_ms, double* ms);
 
 /* h-index bitruss decomposition. Replaces g_bitruss_hindex
  * (bitruss_hidx.cu:179-256), which writes g->bitruss. bitruss_out (m

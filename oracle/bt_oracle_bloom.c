@@ -63,7 +63,7 @@
 #define OR_EINVAL 1
 #define OR_ENOMEM 3
 
-#define DONE 0xFFFFFFFFu   /* edge state: processed                      */
+#define DONE 0xFFu         /* edge state: processed (1, 2: frontier)     */
 #define KWIN 8192u         /* levels per bucket window                    */
 #define SORT_MIN 48u       /* 2-middles from which the rule uses sorted lists */
 
@@ -277,6 +277,28 @@ static inline int64_t find_bloom(const bix* b, uint32_t r, uint32_t w) {
     return -1;
 }
 
+/* Bloom (r, w) or -1, searching r's blooms from *pos on (w ascending over
+ * successive calls): a galloping search that leaves *pos at the first
+ * bloom with far end >= w. */
+static inline int64_t find_bloom_from(const bix* b, uint32_t r, uint32_t w, uint64_t* pos) {
+    uint64_t lo = *pos;
+    uint64_t const end = b->boff[r + 1];
+    uint64_t step = 1, hi = lo;
+    while (hi < end && (b->bw[hi] & 0x7FFFFFFFu) < w) {
+        lo = hi + 1;
+        hi += step;
+        step <<= 1;
+    }
+    if (hi > end) hi = end;
+    while (lo < hi) {
+        uint64_t const mid = (lo + hi) >> 1;
+        if ((b->bw[mid] & 0x7FFFFFFFu) < w) lo = mid + 1;
+        else hi = mid;
+    }
+    *pos = lo;
+    return lo < end && (b->bw[lo] & 0x7FFFFFFFu) == w ? (int64_t)lo : -1;
+}
+
 /* First position in [lo, hi) of anbr with a value > x. */
 static inline uint64_t upper_pos(const uint32_t* a, uint64_t lo, uint64_t hi, uint32_t x) {
     while (lo < hi) {
@@ -346,11 +368,10 @@ typedef struct {
 } counters;
 
 typedef struct {
-    vec32 claimed, touched, next, first;
+    vec32 claimed, touched, next;
     vec32* bucket;  /* KWIN per thread */
     uint32_t *c2a, *c2b, *tmp;
     uint8_t* cls;
-    uint32_t* dec;  /* m: this thread's decrements of the sub-round (plain adds) */
     size_t c2cap;
     counters k;
     int oom;
@@ -373,22 +394,21 @@ typedef struct {
     const or_graph* g;
     bix* b;
     uint64_t* sup;
-    uint32_t* st;      /* 0 live, DONE processed, else frontier sub-round */
-    uint64_t* total;   /* m: the threads' sums of an edge, gathered for the apply */
+    uint8_t* st;       /* 0 live, DONE processed, else the frontier stamp of
+                        * its sub-round (1 and 2 alternate) */
+    uint32_t* dec;     /* m: an edge's decrements in the current sub-round */
     uint32_t* livedeg; /* unprocessed edges per vertex */
     uint32_t* degcur;  /* degree in the working CSR (compaction) */
     uint64_t* S;       /* per vertex v: sum of degcur(w) over live edges wv */
     int flags;         /* OR_BLOOM_* test switches */
 } peel;
 
-/* Per-thread sums without atomics (each thread owns its array); the apply
- * step adds the threads' sums of an edge once. A sub-round takes at most
- * the edge's support from it, below 2^32 (checked at the start). */
+/* Sums in one shared array (relaxed atomic adds; d > 0): the thread whose
+ * add finds the sum at zero lists the edge for the apply step. A sub-round
+ * takes at most the edge's support from it, below 2^32 (checked at the
+ * start). */
 static inline void add_dec(const peel* P, tstate* t, uint32_t e, uint64_t d) {
-    (void)P;
-    uint32_t const o = t->dec[e];
-    t->dec[e] = o + (uint32_t)d;
-    if (o == 0) t->oom |= vpush(&t->touched, e);
+    if (__atomic_fetch_add(&P->dec[e], (uint32_t)d, __ATOMIC_RELAXED) == 0) t->oom |= vpush(&t->touched, e);
 }
 
 /* Claims, for sub-round X, every live bloom that holds frontier edge e,
@@ -408,7 +428,9 @@ static void claim_edge(const peel* P, tstate* t, uint32_t e, uint32_t X) {
             t->oom |= vpush(&t->claimed, (uint32_t)bx_);                                          \
     } while (0)
     /* e = r-v with r = hi: blooms (hi, w), w in N(lo) below hi */
-    for (uint64_t j = b->aoff[lo]; j < b->aoff[lo + 1] && b->anbr[j] < hi; j++) CLAIM(find_bloom(b, hi, b->anbr[j]));
+    uint64_t pos = b->boff[hi];
+    for (uint64_t j = b->aoff[lo]; j < b->aoff[lo + 1] && b->anbr[j] < hi; j++)
+        CLAIM(find_bloom_from(b, hi, b->anbr[j], &pos));
     /* e = v-w with v = lo, w = hi: roots r in N(lo) above hi */
     for (uint64_t j = upper_pos(b->anbr, b->aoff[lo], b->aoff[lo + 1], hi); j < b->aoff[lo + 1]; j++)
         CLAIM(find_bloom(b, b->anbr[j], hi));
@@ -423,7 +445,7 @@ static void claim_edge(const peel* P, tstate* t, uint32_t e, uint32_t X) {
  * 4 dead), prefetching the stamps of middles PF ahead; the decrement and
  * compaction passes read the classes instead of the stamps. */
 #define PF 16
-static void visit_bloom(const peel* P, tstate* t, uint32_t bi, uint32_t X) {
+static void visit_bloom(const peel* P, tstate* t, uint32_t bi, uint8_t X) {
     bix* b = P->b;
     uint64_t* rec = b->rec + b->bstart[bi];
     uint32_t const k = b->blive[bi];
@@ -433,14 +455,14 @@ static void visit_bloom(const peel* P, tstate* t, uint32_t bi, uint32_t X) {
     t->k.mid += k;
     t->k.visits++;
     uint8_t* cls = t->cls;
-    const uint32_t* st = P->st;
+    const uint8_t* st = P->st;
     for (uint32_t i = 0; i < k; i++) {
         if (i + PF < k) {
             __builtin_prefetch(&st[(uint32_t)(rec[i + PF] >> 32)], 0, 0);
             __builtin_prefetch(&st[(uint32_t)rec[i + PF]], 0, 0);
         }
         uint32_t const e1 = (uint32_t)(rec[i] >> 32), e2 = (uint32_t)rec[i];
-        uint32_t const s1 = st[e1], s2 = st[e2];
+        uint8_t const s1 = st[e1], s2 = st[e2];
         uint8_t c;
         if (s1 == DONE || s2 == DONE) c = 4;
         else if (s1 == X && s2 == X) {
@@ -480,8 +502,8 @@ static void visit_bloom(const peel* P, tstate* t, uint32_t bi, uint32_t X) {
         uint64_t const d0 = (uint64_t)n1 + n2;
         for (uint32_t i = 0; i < k; i++) {
             if (i + PF < k && cls[i + PF] <= 2) {
-                __builtin_prefetch(&t->dec[(uint32_t)(rec[i + PF] >> 32)], 1, 0);
-                __builtin_prefetch(&t->dec[(uint32_t)rec[i + PF]], 1, 0);
+                __builtin_prefetch(&P->dec[(uint32_t)(rec[i + PF] >> 32)], 1, 0);
+                __builtin_prefetch(&P->dec[(uint32_t)rec[i + PF]], 1, 0);
             }
             uint8_t const c = cls[i];
             if (c > 2) continue;
@@ -551,19 +573,18 @@ int or_msp_bloom_ck(const or_graph* g, int threads, uint64_t max_level, int flag
     bix b;
     memset(&b, 0, sizeof(b));
     peel P = {g, &b, state, NULL, NULL, NULL, NULL, NULL, flags};
-    P.st = (uint32_t*)calloc(m ? m : 1, sizeof(uint32_t));
-    P.total = (uint64_t*)calloc(m ? m : 1, sizeof(uint64_t));
+    P.st = (uint8_t*)calloc(m ? m : 1, 1);
+    P.dec = (uint32_t*)calloc(m ? m : 1, sizeof(uint32_t));
     P.livedeg = (uint32_t*)calloc(n ? n : 1, sizeof(uint32_t));
     P.degcur = (uint32_t*)calloc(n ? n : 1, sizeof(uint32_t));
     P.S = (uint64_t*)calloc(n ? n : 1, sizeof(uint64_t));
     uint32_t* fr = (uint32_t*)malloc(sizeof(uint32_t) * (m ? m : 1));   /* frontier */
     uint32_t* cl = NULL;                                                /* claimed blooms */
     tstate* ts = (tstate*)calloc((size_t)threads, sizeof(tstate));
-    if (!P.st || !P.total || !P.livedeg || !P.degcur || !P.S || !fr || !ts) { rc = OR_ENOMEM; goto out; }
+    if (!P.st || !P.dec || !P.livedeg || !P.degcur || !P.S || !fr || !ts) { rc = OR_ENOMEM; goto out; }
     for (int t = 0; t < threads; t++) {
         ts[t].bucket = (vec32*)calloc(KWIN, sizeof(vec32));
-        ts[t].dec = (uint32_t*)calloc(m ? m : 1, sizeof(uint32_t));
-        if (!ts[t].bucket || !ts[t].dec) { rc = OR_ENOMEM; goto out; }
+        if (!ts[t].bucket) { rc = OR_ENOMEM; goto out; }
     }
     for (uint32_t e = 0; e < m; e++)
         if (state[e] >> 32) { rc = OR_EINVAL; goto out; } /* supports beyond the u32 sums */
@@ -653,7 +674,7 @@ int or_msp_bloom_ck(const or_graph* g, int threads, uint64_t max_level, int flag
             compact_counts(&P, threads);
             mcur = live;
         }
-        for (uint32_t i = 0; i < nfr; i++) P.st[fr[i]] = X;
+        for (uint32_t i = 0; i < nfr; i++) P.st[fr[i]] = (uint8_t)(1 + (X & 1));
         /* ---- sub-rounds of level L */
         while (nfr > 0) {
             live -= nfr;
@@ -661,7 +682,7 @@ int or_msp_bloom_ck(const or_graph* g, int threads, uint64_t max_level, int flag
 #pragma omp parallel num_threads(threads)
             {
                 tstate* t = &ts[omp_get_thread_num()];
-                t->claimed.n = t->touched.n = t->next.n = t->first.n = 0;
+                t->claimed.n = t->touched.n = t->next.n = 0;
                 double const tq0 = progress ? omp_get_wtime() : 0.0;
 #pragma omp for schedule(dynamic, 16)
                 for (int64_t i = 0; i < (int64_t)nfr; i++) claim_edge(&P, t, fr[i], X);
@@ -675,27 +696,19 @@ int or_msp_bloom_ck(const or_graph* g, int threads, uint64_t max_level, int flag
                 }
                 double const tv0 = progress ? omp_get_wtime() : 0.0;
 #pragma omp for schedule(dynamic, 1)
-                for (int64_t i = 0; i < (int64_t)ncl; i++) visit_bloom(&P, t, cl[i], X);
+                for (int64_t i = 0; i < (int64_t)ncl; i++) visit_bloom(&P, t, cl[i], (uint8_t)(1 + (X & 1)));
                 if (progress && omp_get_thread_num() == 0) t_visit += omp_get_wtime() - tv0;
                 double const ta0 = progress ? omp_get_wtime() : 0.0;
-                /* gather the threads' sums: the thread whose add finds the
-                 * total at zero applies the edge */
+                /* s' = max(L, s - d) for the edges this thread listed (the
+                 * visits ended at the loop's barrier); the edges that reach
+                 * L are the next sub-round's frontier */
                 for (size_t i = 0; i < t->touched.n; i++) {
                     uint32_t const e = t->touched.a[i];
-                    if (__atomic_fetch_add(&P.total[e], (uint64_t)t->dec[e], __ATOMIC_RELAXED) == 0)
-                        t->oom |= vpush(&t->first, e);
-                    t->dec[e] = 0;
-                }
-#pragma omp barrier
-                /* s' = max(L, s - d); the edges that reach L are the next
-                 * sub-round's frontier */
-                for (size_t i = 0; i < t->first.n; i++) {
-                    uint32_t const e = t->first.a[i];
-                    uint64_t const d = P.total[e], s = state[e];
-                    P.total[e] = 0;
+                    uint64_t const d = P.dec[e], s = state[e];
+                    P.dec[e] = 0;
                     if (s - L <= d) {
                         state[e] = L;
-                        P.st[e] = X + 1;
+                        P.st[e] = (uint8_t)(1 + ((X + 1) & 1));
                         t->oom |= vpush(&t->next, e);
                     } else {
                         state[e] = s - d;
@@ -765,15 +778,15 @@ int or_msp_bloom_ck(const or_graph* g, int threads, uint64_t max_level, int flag
 out:
     if (ts) {
         for (int t = 0; t < threads; t++) {
-            free(ts[t].claimed.a); free(ts[t].touched.a); free(ts[t].next.a); free(ts[t].first.a);
+            free(ts[t].claimed.a); free(ts[t].touched.a); free(ts[t].next.a);
             if (ts[t].bucket)
                 for (uint32_t i = 0; i < KWIN; i++) free(ts[t].bucket[i].a);
             free(ts[t].bucket);
-            free(ts[t].c2a); free(ts[t].c2b); free(ts[t].tmp); free(ts[t].cls); free(ts[t].dec);
+            free(ts[t].c2a); free(ts[t].c2b); free(ts[t].tmp); free(ts[t].cls);
         }
         free(ts);
     }
-    free(P.st); free(P.total); free(P.livedeg); free(P.degcur); free(P.S); free(fr); free(cl);
+    free(P.st); free(P.dec); free(P.livedeg); free(P.degcur); free(P.S); free(fr); free(cl);
     bix_free(&b);
     return rc;
 }
