@@ -76,9 +76,6 @@ constexpr int kXCap = 2048;             // frontier edges buffered per wave
 constexpr int kBL = 1024;               // frontier edges per block claim batch
 constexpr uint32_t kBlkCap = 16384;     // claimed blooms a block keeps in its own list
 constexpr uint32_t kBigCap = 64;        // large blooms a block walks together per sub-round
-constexpr uint32_t kGBig = 512;         // blooms of a sub-round's shared large-bloom list
-constexpr uint32_t kChunk = 2 * kMB;    // middles of one chunk of a shared large bloom
-constexpr uint32_t kGBigCnt = 8;        // u32 counters per shared large bloom
 constexpr uint32_t kWindowEdges = 4u << 20;  // live edges from which the collect uses a window
 constexpr int kSp = 32;                 // u32 spacing of the barrier slots (128 B)
 constexpr uint64_t kInf = ~0ull;
@@ -127,10 +124,6 @@ struct MspCtl {
     // work (max over blocks, summed by block 0), middles walked by waves
     // and by whole blocks, large-bloom visits
     unsigned long long cmax[2], cmax_sum, wmax[2], wmax_sum, mid_wave, mid_block, n_block;
-    // shared large blooms of sub-round Y (by Y % 3): list length, and the
-    // grab counters of their counting and decrement chunks
-    unsigned big_n[4], big_qc[4], big_qd[4];
-    unsigned long long mid_split, n_split;  // diagnostics: middles and blooms split over blocks
 };
 static_assert(sizeof(MspCtl) <= 128 * sizeof(uint64_t), "control block overlays dg->ctl");
 
@@ -180,25 +173,7 @@ struct MspArgs {
     // mspp reads the orientation flag of each edge from flipbits
     uint32_t rule;
     const uint32_t* flipbits;
-    // blooms claimed with more middles than split_min (0: never) go to the
-    // sub-round's shared list and are walked in chunks by every block
-    // (gbig: per rotation kGBig first records, kGBig * kGBigCnt counters,
-    // kGBig * kC2Cap both-frontier middles)
-    uint32_t split_min;
-    uint32_t* gbig;
 };
-
-// The shared large-bloom list of rotation r (sub-round % 3).
-struct BigList {
-    uint32_t* ent;   // [kGBig] first record of each bloom
-    uint32_t* cnt;   // [kGBig][kGBigCnt]: n0, n1, n2, n_p, counted chunks
-    uint64_t* c2;    // [kGBig][kC2Cap] both-frontier middles
-};
-constexpr size_t kGBigU32 = size_t(kGBig) * (1 + kGBigCnt + 2 * kC2Cap);  // u32 per rotation
-__device__ __forceinline__ BigList big_list(const MspArgs& a, uint32_t r) {
-    uint32_t* base = a.gbig + size_t(r) * kGBigU32;
-    return BigList{base, base + kGBig, reinterpret_cast<uint64_t*>(base + kGBig * (1 + kGBigCnt))};
-}
 
 // relaxed device-scope accesses (coherent across XCDs)
 template <class T>
@@ -319,26 +294,10 @@ struct Out {
     uint32_t* shared;     // ovf + Y % 3
     unsigned* shared_n;   // ctl->ovf_n + Y % 3
     unsigned* cnt;        // LDS counter
-    uint32_t* big_ent;    // shared large-bloom list of Y (null: none)
-    unsigned* big_n;      // ctl->big_n + Y % 3
-    // appends, for one wave, the entries v[u] of the lanes with app[u];
-    // with big set, a lane whose bloom is large (big[u]) puts it in the
-    // shared large-bloom list instead while that list has room
+    // appends, for one wave, the entries v[u] of the lanes with app[u]
     template <int U>
-    __device__ __forceinline__ void append(bool* app, const uint32_t* v, const bool* big = nullptr) const {
+    __device__ __forceinline__ void append(const bool* app, const uint32_t* v) const {
         int const lane = threadIdx.x & (kWave - 1);
-        if (big) {
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                if (app[u] && big[u]) {
-                    unsigned const pos = afetch_add(big_n, 1u);
-                    if (pos < kGBig) {
-                        ast(&big_ent[pos], v[u]);
-                        app[u] = false;
-                    }
-                }
-            }
-        }
         uint64_t bal[U];
         unsigned tot = 0;
 #pragma unroll
@@ -412,19 +371,13 @@ __device__ __forceinline__ void claim_edges(const uint32_t* xl, uint32_t n, uint
                 sb[u] = a.inc[q + ws.off[j]];
                 sc[u] = qr < total ? sb[u] : dummy_claim(a, u);
             }
-            uint32_t old[kClaimUnroll], hb[kClaimUnroll];
+            uint32_t old[kClaimUnroll];
 #pragma unroll
             for (int u = 0; u < kClaimUnroll; u++) old[u] = afetch_max(&a.claim[sc[u]], Y);
-            // bloom sizes, loaded beside the claims (the shared large-bloom list)
+            bool app[kClaimUnroll];
 #pragma unroll
-            for (int u = 0; u < kClaimUnroll; u++) hb[u] = a.split_min ? ald(&a.hdr[sb[u]]) : 0u;
-            bool app[kClaimUnroll], big[kClaimUnroll];
-#pragma unroll
-            for (int u = 0; u < kClaimUnroll; u++) {
-                app[u] = q0 + u * kWave + lane < total && old[u] < Y;
-                big[u] = a.split_min && (hb[u] & 0x7FFFFFFFu) > a.split_min;
-            }
-            out.append<kClaimUnroll>(app, sb, big);
+            for (int u = 0; u < kClaimUnroll; u++) app[u] = q0 + u * kWave + lane < total && old[u] < Y;
+            out.append<kClaimUnroll>(app, sb);
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -507,18 +460,13 @@ __device__ __forceinline__ void claim_block(const MspArgs& a, uint32_t Y, const 
                 sb[u] = a.inc[q + s_off[j]];
                 sc[u] = qr < total ? sb[u] : dummy_claim(a, u);
             }
-            uint32_t old[kClaimUnroll], hb[kClaimUnroll];
+            uint32_t old[kClaimUnroll];
 #pragma unroll
             for (int u = 0; u < kClaimUnroll; u++) old[u] = afetch_max(&a.claim[sc[u]], Y);
+            bool app[kClaimUnroll];
 #pragma unroll
-            for (int u = 0; u < kClaimUnroll; u++) hb[u] = a.split_min ? ald(&a.hdr[sb[u]]) : 0u;
-            bool app[kClaimUnroll], big[kClaimUnroll];
-#pragma unroll
-            for (int u = 0; u < kClaimUnroll; u++) {
-                app[u] = q0 + u * kMB + tid < total && old[u] < Y;
-                big[u] = a.split_min && (hb[u] & 0x7FFFFFFFu) > a.split_min;
-            }
-            out.append<kClaimUnroll>(app, sb, big);
+            for (int u = 0; u < kClaimUnroll; u++) app[u] = q0 + u * kMB + tid < total && old[u] < Y;
+            out.append<kClaimUnroll>(app, sb);
         }
         __syncthreads();
     }
@@ -962,12 +910,8 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
     if (a.slice_ticks && B == 0 && tid == 0) ctl->t_launch = wall_clock64();
     // claim list of sub-round Y filled by this block
     auto out_for = [&](uint32_t Y) {
-        return Out{a.area + (size_t(Y % 3) * G + B) * kBlkCap,
-                   a.ovf + size_t(Y % 3) * a.nbloom,
-                   &ctl->ovf_n[Y % 3],
-                   &s_on,
-                   a.split_min ? big_list(a, Y % 3).ent : nullptr,
-                   &ctl->big_n[Y % 3]};
+        return Out{a.area + (size_t(Y % 3) * G + B) * kBlkCap, a.ovf + size_t(Y % 3) * a.nbloom, &ctl->ovf_n[Y % 3],
+                   &s_on};
     };
     // publishes this block's claim-list length for sub-round Y (whole block)
     auto close_out = [&](uint32_t Y) {
