@@ -45,6 +45,10 @@ def test_graph_rounds_equal_rounds_one_by_one(dev):
 
 @pytest.mark.gpu
 def test_graph_rounds_in_batches(dev):
+    """A batch union with graph rounds: every graph's results and round
+    count equal its plain run with rounds one by one (the plain runs are
+    checked against the oracle above and in the parity tests), and the
+    oracle's on the first three graphs."""
     graphs = [gb.Graph.generate("mix_graph", i) for i in range(12)]
     with gb.Device(dev.device) as d:
         d.set_hidx_graph_rounds(4)
@@ -52,6 +56,12 @@ def test_graph_rounds_in_batches(dev):
         say(f"batch of {len(graphs)} mix graphs, 4 rounds per graph")
         outs, results = b.run("hidx")
         b.free()
+        d.set_hidx_graph_rounds(0)
         for i, g in enumerate(graphs):
-            want, wrounds, _ = oracle.hidx(g, 8)
+            dg = d.upload(g)
+            want, wrounds = dg.hidx()
+            dg.free()
             assert np.array_equal(outs[i], want) and results[i]["rounds"] == wrounds, i
+            if i < 3:
+                w2, r2, _ = oracle.hidx(g, 8)
+                assert np.array_equal(w2, want) and r2 == wrounds, i

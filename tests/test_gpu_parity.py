@@ -385,7 +385,12 @@ def test_skewed_60m_msp_prefix(dev):
     # the per-pivot oracle's 800-level state, then the bloom oracle's deeper
     # checkpoints (its 800-level state is the per-pivot one)
     want = {gold["msp_prefix_levels"]: gold["msp_prefix_sha256"]}
-    want.update({int(k): v for k, v in gold.get("msp_prefix_checkpoints", {}).items()})
+    ck = {int(k): v for k, v in gold.get("msp_prefix_checkpoints", {}).items()}
+    # every checkpoint up to 3,200 levels and the two deepest (each run
+    # peels from the start)
+    for level in sorted(ck):
+        if level <= 3200 or level in sorted(ck)[-2:]:
+            want[level] = ck[level]
     with gb.Device(dev.device) as d:
         # the bloom-index peel or a failure: never the merge path, which
         # would take hours here
@@ -398,3 +403,21 @@ def test_skewed_60m_msp_prefix(dev):
             assert dg.partial
             assert digest(res) == want[level], level
         dg.free()
+
+
+@pytest.mark.gpu
+def test_skewed_60m_msp_whole(dev):
+    """The 60M-edge graph's whole msp decomposition against the bloom
+    oracle's digest and level count (or_msp_bloom, cpu_bitruss.cpp:536-597),
+    when the oracle's run has been recorded."""
+    gold = json.load(open(os.path.join(GOLDEN, "skewed_60m.json")))
+    if "msp_sha256" not in gold:
+        pytest.skip("the oracle's whole msp of skewed_60m is not recorded (make_golden.py --msp-bloom)")
+    g = gb.Graph.generate("skewed_60m")
+    with gb.Device(dev.device) as d:
+        d.set_msp_path("index")
+        dg = d.upload(g)
+        say("skewed_60m whole msp")
+        res, levels = dg.msp()
+        dg.free()
+    assert digest(res) == gold["msp_sha256"] and levels == gold["msp_levels"]
