@@ -207,13 +207,24 @@ int chunk_run(bt_ctx* ctx, bt_chunk& c, int algo, uint64_t* const* outs, bt_batc
 
 }  // namespace
 
+void bt_batch_detach(bt_batch* b) {
+    if (b->ctx) (void)hipSetDevice(b->ctx->device);
+    for (bt_chunk& c : b->chunks) chunk_free(c, b->ctx);
+    b->chunks.clear();
+    for (bt_ctx* lc : b->lane_ctx) bt_ctx_destroy(lc);
+    b->lane_ctx.clear();
+    b->ctx = nullptr;
+}
+
 extern "C" {
 
 int bt_batch_free(bt_batch* b) {
     if (!b) return BT_ERR_INVALID;
-    if (b->ctx) (void)hipSetDevice(b->ctx->device);
-    for (bt_chunk& c : b->chunks) chunk_free(c, b->ctx);
-    for (bt_ctx* lc : b->lane_ctx) bt_ctx_destroy(lc);
+    if (b->ctx) {
+        std::lock_guard<std::mutex> lk(b->ctx->live_mu);
+        b->ctx->live_batches.erase(b);
+    }
+    bt_batch_detach(b);
     delete b;
     return BT_OK;
 }
@@ -289,6 +300,10 @@ int bt_batch_upload(bt_ctx* ctx, const bt_host_graph* graphs, size_t k, bt_batch
             bt_batch_free(b);
             return st;
         }
+    }
+    {
+        std::lock_guard<std::mutex> lk(ctx->live_mu);
+        ctx->live_batches.insert(b);
     }
     *out = b;
     return BT_OK;

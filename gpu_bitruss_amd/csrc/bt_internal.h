@@ -9,6 +9,7 @@
 #include <utility>
 #include <vector>
 #include <map>
+#include <set>
 
 #include <cstddef>
 #include <cstdint>
@@ -122,7 +123,17 @@ struct bt_ctx {
     std::mutex cache_mu;
     std::multimap<size_t, void*> cache;
     size_t cache_bytes = 0;
+    // graphs and batches made on this context and not freed yet: destroying
+    // the context releases their device memory and detaches them, so that
+    // freeing them afterwards only deletes the handle
+    std::mutex live_mu;
+    std::set<bt_dgraph*> live_graphs;
+    std::set<bt_batch*> live_batches;
 };
+
+// Releases a batch's device memory and lane contexts and detaches it from
+// its context (bt_ctx_destroy of that context; bt_batch_free then deletes it).
+void bt_batch_detach(bt_batch* b);
 
 // Device memory for graphs and batches from the context's cache of freed
 // blocks (best fit within twice the request), else hipMalloc; *got is the

@@ -1112,7 +1112,7 @@ __global__ __launch_bounds__(kMB) void k_msp_persist(MspArgs a) {
         // live middles; middles keep no order.
         __syncthreads();
         uint32_t const nbig = s_nbig < kBigCap ? s_nbig : kBigCap;
-        constexpr int UB = 2;
+        constexpr int UB = 4;
         for (uint32_t bi = 0; bi < nbig; bi++) {
             uint32_t const s = s_big[bi];
             uint32_t const hdr = ald(&a.hdr[s]);

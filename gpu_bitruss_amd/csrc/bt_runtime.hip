@@ -253,6 +253,27 @@ int bt_ctx_destroy(bt_ctx* c) {
     if (!c) return BT_ERR_INVALID;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    // batches and graphs still alive: their memory goes back with the
+    // context's, and their handles stay valid for bt_*_free (which then
+    // only deletes them)
+    std::vector<bt_batch*> batches;
+    {
+        std::lock_guard<std::mutex> lk(c->live_mu);
+        batches.assign(c->live_batches.begin(), c->live_batches.end());
+        c->live_batches.clear();
+    }
+    for (bt_batch* b : batches) bt_batch_detach(b);
+    std::vector<bt_dgraph*> graphs;
+    {
+        std::lock_guard<std::mutex> lk(c->live_mu);
+        graphs.assign(c->live_graphs.begin(), c->live_graphs.end());
+        c->live_graphs.clear();
+    }
+    for (bt_dgraph* dg : graphs) {
+        for (auto const& b : dg->blocks) bt_ctx_release(c, b.first, b.second);
+        dg->blocks.clear();
+        dg->ctx = nullptr;
+    }
     bt_buf_free(c->sort_tmp);
     bt_buf_free(c->pool);
     bt_buf_free(c->pinned_dummy);
@@ -352,6 +373,8 @@ int bt_graph_free(bt_dgraph* dg) {
     if (dg->ctx) {
         (void)hipSetDevice(dg->ctx->device);
         (void)hipStreamSynchronize(dg->ctx->stream);
+        std::lock_guard<std::mutex> lk(dg->ctx->live_mu);
+        dg->ctx->live_graphs.erase(dg);
     }
     for (auto const& b : dg->blocks) bt_ctx_release(dg->ctx, b.first, b.second);
     delete dg;
@@ -418,6 +441,10 @@ int bt_graph_upload(bt_ctx* ctx, const bt_host_graph* g, bt_dgraph** out) {
     if (st != BT_OK) {
         bt_graph_free(dg);
         return st;
+    }
+    {
+        std::lock_guard<std::mutex> lk(ctx->live_mu);
+        ctx->live_graphs.insert(dg);
     }
     *out = dg;
     return BT_OK;
