@@ -280,19 +280,41 @@ __device__ __forceinline__ uint64_t expand_entries(const RowView& g, uint32_t r,
     return tot;
 }
 
-// Buckets A and B: one root per workgroup iteration, table in LDS.
+// Device-wide work queue: the workgroup's next run of `grab` units of
+// [0, count) from the counter q (every workgroup takes runs until the
+// counter passes count), so that uneven units spread over the workgroups as
+// they finish. Returns the run's first unit (>= count: none left).
+template <int BLOCK>
+__device__ __forceinline__ uint32_t take_units(unsigned* q, uint32_t grab, uint32_t* s_take) {
+    if (threadIdx.x == 0) *s_take = atomicAdd(q, grab);
+    __syncthreads();
+    uint32_t const first = *s_take;
+    __syncthreads();
+    return first;
+}
+
+// Buckets A and B: one root per workgroup iteration, table in LDS; roots
+// come from a device-wide queue in runs of GRAB.
 template <int BLOCK, uint32_t CAP>
 __global__ __launch_bounds__(BLOCK) void k_count_lds(const uint32_t* __restrict__ roots, uint32_t nroots,
                                                      RowView g, const uint32_t* __restrict__ r_lo,
                                                      uint64_t* __restrict__ sup, uint64_t* __restrict__ total,
-                                                     int cb, int per_edge, Emit em) {
+                                                     int cb, int per_edge, Emit em, unsigned* queue) {
     __shared__ uint32_t tab[CAP];
     __shared__ ExpandShared<BLOCK> sh;
+    __shared__ uint32_t s_take;
     constexpr int kLog = __builtin_ctz(CAP);
+    constexpr uint32_t GRAB = BLOCK == kWave ? 16u : 2u;  // wave roots are light
     LdsHash h{tab, CAP - 1, 32 - kLog, cb};
     uint64_t tot = 0;
-    for (uint32_t ri = blockIdx.x; ri < nroots; ri += gridDim.x) {
-        uint32_t const r = roots[ri];
+    uint32_t ri = 0, rend = 0;
+    for (;;) {
+        if (ri == rend) {
+            ri = take_units<BLOCK>(queue, GRAB, &s_take);
+            if (ri >= nroots) break;
+            rend = ri + GRAB < nroots ? ri + GRAB : nroots;
+        }
+        uint32_t const r = roots[ri++];
         uint32_t const b = g.r_off[r], lo = r_lo[r];
         for (uint32_t i = threadIdx.x; i < CAP; i += BLOCK) tab[i] = 0;
         __syncthreads();
@@ -315,10 +337,13 @@ template <int BLOCK, int PASS>
 __global__ __launch_bounds__(BLOCK) void k_count_dense(const HubItem* __restrict__ items, uint32_t nitems,
                                                        RowView g, uint32_t* __restrict__ pool,
                                                        uint64_t* __restrict__ sup, uint64_t* __restrict__ total,
-                                                       Emit em) {
+                                                       Emit em, unsigned* queue) {
     __shared__ ExpandShared<BLOCK> sh;
+    __shared__ uint32_t s_take;
     uint64_t tot = 0;
-    for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    for (;;) {
+        uint32_t const it = take_units<BLOCK>(queue, 1u, &s_take);
+        if (it >= nitems) break;
         HubItem const h = items[it];
         DenseTable t{pool + h.toff};
         tot += expand_entries<BLOCK, PASS>(g, h.r, h.i0, h.i1, t, sup, em, sh);
@@ -356,10 +381,11 @@ struct LdsLaunch {
     Emit em;
     unsigned grid_cap;
     hipStream_t s;
+    unsigned* queue;  // zeroed device word: the roots taken so far
     template <int BLOCK, uint32_t CAP>
     void run() const {
         k_count_lds<BLOCK, CAP><<<grid_for(nroots, 1, grid_cap), BLOCK, 0, s>>>(roots, nroots, g, r_lo, sup, total,
-                                                                               cb, per_edge, em);
+                                                                               cb, per_edge, em, queue);
     }
     int bucket_a(uint32_t wave_wedges) const {
         switch (wave_wedges) {
@@ -393,11 +419,12 @@ struct LdsLaunch {
     }
 };
 
+// queues: two zeroed device words, one per pass
 template <int BLOCK>
 void launch_dense(unsigned grid, hipStream_t s, const HubItem* items, uint32_t nitems, const RowView& g,
-                  uint32_t* pool, uint64_t* sup, uint64_t* total, const Emit& em, bool per_edge) {
-    k_count_dense<BLOCK, 1><<<grid, BLOCK, 0, s>>>(items, nitems, g, pool, sup, total, em);
-    if (per_edge) k_count_dense<BLOCK, 2><<<grid, BLOCK, 0, s>>>(items, nitems, g, pool, sup, total, em);
+                  uint32_t* pool, uint64_t* sup, uint64_t* total, const Emit& em, bool per_edge, unsigned* queues) {
+    k_count_dense<BLOCK, 1><<<grid, BLOCK, 0, s>>>(items, nitems, g, pool, sup, total, em, queues);
+    if (per_edge) k_count_dense<BLOCK, 2><<<grid, BLOCK, 0, s>>>(items, nitems, g, pool, sup, total, em, queues + 1);
 }
 
 }  // namespace
@@ -518,11 +545,14 @@ int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total_out, uint64_t* s
     uint64_t* total = dg->ctl + 8;
     unsigned const max_grid = unsigned(c->cu_count) * 16;
     BT_TRY(tm.start(2));
+    // work-queue counters in control words 4..7 (zeroed above): bucket A,
+    // bucket B, the hub items' two passes (reset per batch of hub roots)
+    unsigned* queues = reinterpret_cast<unsigned*>(dg->ctl + 4);
     if (na)
-        BT_TRY((LdsLaunch{la, uint32_t(na), g, dg->r_lo, dg->sup, total, cb, per_edge, em, max_grid * 4, s}
+        BT_TRY((LdsLaunch{la, uint32_t(na), g, dg->r_lo, dg->sup, total, cb, per_edge, em, max_grid * 4, s, queues}
                     .bucket_a(cfg.wave_wedges)));
     if (nb)
-        BT_TRY((LdsLaunch{lb, uint32_t(nb), g, dg->r_lo, dg->sup, total, cb, per_edge, em, max_grid, s}
+        BT_TRY((LdsLaunch{lb, uint32_t(nb), g, dg->r_lo, dg->sup, total, cb, per_edge, em, max_grid, s, queues + 2}
                     .bucket_b(cfg.block_wedges, cfg.block_threads)));
     BT_HIP_TRY(hipGetLastError());
     if (nc) {
@@ -568,12 +598,15 @@ int bt_count_impl(bt_dgraph* dg, bool per_edge, uint64_t* total_out, uint64_t* s
             }
             if (k2 == k || items.size() > items_cap) return BT_ERR_INTERNAL;
             BT_HIP_TRY(hipMemsetAsync(pool, 0, used * sizeof(uint32_t), s));
+            BT_HIP_TRY(hipMemsetAsync(queues + 4, 0, 2 * sizeof(unsigned), s));
             BT_HIP_TRY(hipMemcpyAsync(ditems, items.data(), items.size() * sizeof(HubItem), hipMemcpyHostToDevice, s));
             unsigned const gC = grid_for(items.size(), 1, max_grid);
             if (cfg.hub_threads == 512)
-                launch_dense<512>(gC, s, ditems, uint32_t(items.size()), g, pool, dg->sup, total, em, per_edge);
+                launch_dense<512>(gC, s, ditems, uint32_t(items.size()), g, pool, dg->sup, total, em, per_edge,
+                                  queues + 4);
             else
-                launch_dense<256>(gC, s, ditems, uint32_t(items.size()), g, pool, dg->sup, total, em, per_edge);
+                launch_dense<256>(gC, s, ditems, uint32_t(items.size()), g, pool, dg->sup, total, em, per_edge,
+                                  queues + 4);
             BT_HIP_TRY(hipGetLastError());
             // the host vector `items` is reused only after this batch's copy
             BT_HIP_TRY(hipStreamSynchronize(s));
