@@ -414,6 +414,20 @@ def main() -> int:
             "sample": f"powerlaw_2m: support count (W_cnt {st['w_cnt']}) plus {done} of "
                       f"{gold['msp_levels']} peel levels (W_peel {st['w_peel']} of {stats['w_peel']}), "
                       f"{secs:.1f} s on {threads} threads",
+            # the host's CPUs as the OS reports them; a run uses at most 16
+            # threads (the share of the host that goes with one GPU)
+            "host_cpus": os.cpu_count(),
+        }
+        # the same oracle on one thread, over a shorter bound
+        log(f"cpu baseline: oracle msp sample, 1 thread, {args.cpu_seconds / 2:.0f} s")
+        secs1, clev1, complete1, st1 = cpu_baseline(g, 1, args.cpu_seconds / 2)
+        done1 = "all" if complete1 else f"the first {clev1}"
+        out["cpu_baseline"]["single_thread"] = {
+            "value": (st1["w_cnt"] + st1["w_peel"]) / secs1,
+            "unit": "wedges/s",
+            "cores": 1,
+            "sample": f"powerlaw_2m: support count plus {done1} of {gold['msp_levels']} peel levels "
+                      f"(W_peel {st1['w_peel']}), {secs1:.1f} s on 1 thread",
         }
     dg.free()
     dev.close()
