@@ -74,6 +74,9 @@ int bt_device_info(int device, bt_device_props* out);
 /* Create a context on `device` (main.cu:58-65 device selection): one
  * HIP stream and a reusable scratch pool. */
 int bt_ctx_create(int device, bt_ctx** out);
+/* Destroying a context gives back the device memory of the graphs and
+ * batches still uploaded on it and detaches them: their handles stay valid
+ * only for bt_graph_free / bt_batch_free, which then just delete them. */
 int bt_ctx_destroy(bt_ctx* ctx);
 int bt_ctx_last_timings(const bt_ctx* ctx, bt_timings* out);
 /* Give back the device memory the context keeps between calls: the blocks
