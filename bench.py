@@ -188,7 +188,7 @@ def bench_mix(args) -> int:
     return 0
 
 
-def serve_levels(n: int, qps_list, serve_cus: str, batch_wedges: int):
+def serve_levels(n: int, qps_list, serve_cus: str, batch_wedges: int, light_workers: int = 0):
     """Poisson request stream (seed 7): 98% graphs of the mix distribution,
     2% large Chung-Lu requests; p50/p99 latency and graphs/s per offered QPS,
     against the plain path on the same requests one after the other (upload
@@ -218,7 +218,7 @@ def serve_levels(n: int, qps_list, serve_cus: str, batch_wedges: int):
     levels = []
     for qps in qps_list:
         gaps = rng.exponential(1.0 / qps, n)
-        with gb.Server(large_edges=1_000_000, batch_wedges=batch_wedges, cus=cus) as srv:
+        with gb.Server(large_edges=1_000_000, batch_wedges=batch_wedges, cus=cus, light_workers=light_workers) as srv:
             tickets = [None] * n
             start = time.perf_counter()
 
@@ -242,12 +242,14 @@ def serve_levels(n: int, qps_list, serve_cus: str, batch_wedges: int):
         levels.append({"qps": qps, "p50_ms": p50, "p99_ms": p99, "graphs_per_s": n / wall,
                        "batches": st["batches"], "large_runs": st["large_runs"]})
     return {"requests": n, "large": int(large.sum()), "plain_graphs_per_s": n / t_plain, "levels": levels,
-            "lane_cus": list(cus), "batch_wedges": batch_wedges or 100_000_000}
+            "lane_cus": list(cus), "batch_wedges": batch_wedges or 100_000_000,
+            "light_workers": light_workers or 2}
 
 
 def bench_serve(args) -> int:
     n = args.requests
-    sv = serve_levels(n, [float(q) for q in args.qps.split(",")], args.serve_cus, args.serve_batch_wedges)
+    sv = serve_levels(n, [float(q) for q in args.qps.split(",")], args.serve_cus, args.serve_batch_wedges,
+                      args.serve_light_workers)
     levels, cus = sv["levels"], sv["lane_cus"]
     best = max(levels, key=lambda x: x["graphs_per_s"])
     print(json.dumps({
@@ -283,6 +285,8 @@ def main() -> int:
     ap.add_argument("--lanes", type=int, default=0, help="mix: batch lanes (0: the library default, 2)")
     ap.add_argument("--requests", type=int, default=400, help="serve: requests per offered rate")
     ap.add_argument("--qps", default="25,100,400", help="serve: offered request rates")
+    ap.add_argument("--serve-light-workers", type=int, default=0,
+                    help="workers of the serving light lane (0: the library's default, 2)")
     ap.add_argument("--serve-cus", default="0,0,0",
                     help="serve: CUs of the light, heavy and large lanes (0: the library defaults)")
     ap.add_argument("--serve-batch-wedges", type=int, default=0,
@@ -439,7 +443,8 @@ def main() -> int:
         log(f"batched {bt['batched_graphs_per_s']:.1f} graphs/s, plain {bt['plain_graphs_per_s']:.1f}, "
             f"x{bt['speedup']:.2f}, identical {bt['identical']} of {bt['identical_of']}")
     if wl == WORKLOAD and not args.no_serve:
-        out["serve"] = serve_levels(400, [args.serve_qps], args.serve_cus, args.serve_batch_wedges)
+        out["serve"] = serve_levels(400, [args.serve_qps], args.serve_cus, args.serve_batch_wedges,
+                                    args.serve_light_workers)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -86,6 +86,7 @@ class BtServeConfig(C.Structure):
         ("light_cus", C.c_uint32),
         ("heavy_cus", C.c_uint32),
         ("large_cus", C.c_uint32),
+        ("light_workers", C.c_uint32),
     ]
 
 

@@ -15,6 +15,10 @@
 //     way (a union of large graphs costs about as many phases as its
 //     largest graph, so concurrent large requests finish together instead
 //     of one after the other).
+// The light lane has two workers, each with its own context: while one
+// worker's batch runs, requests that arrive are taken by the other, whose
+// copy to the device, support count and bloom index overlap the running
+// peel (its own peel starts when the CU lease has room).
 // Each lane has its own context (HIP stream); the lanes' kernels overlap on
 // the device, and their persistent peels run side by side on disjoint CU
 // shares (bt_cu_lease; a quarter for the light lane, three eighths for the
@@ -82,9 +86,10 @@ struct bt_server {
     bool stopping = false;
     bt_serve_stats stats{};
     bt_ctx* ctx_light = nullptr;
+    bt_ctx* ctx_light2 = nullptr;  // the light lane's second worker (may stay null)
     bt_ctx* ctx_heavy = nullptr;
     bt_ctx* ctx_large = nullptr;
-    std::thread thr_light, thr_heavy, thr_large;
+    std::thread thr_light, thr_light2, thr_heavy, thr_large;
 
     // takes the next group of requests from q that fits the budget (caller
     // holds mu); empty when the front does not fit yet
@@ -187,7 +192,13 @@ int bt_serve_start(const bt_serve_config* cfg, bt_server** out) {
     if (!s->cfg.max_batch) s->cfg.max_batch = 1024;
     if (!s->cfg.light_wedges) s->cfg.light_wedges = 2000000;
     if (!s->cfg.batch_wedges) s->cfg.batch_wedges = 100000000;
+    if (!s->cfg.light_workers) s->cfg.light_workers = 2;
+    if (s->cfg.light_workers > 2) {
+        delete s;
+        return BT_ERR_INVALID;
+    }
     int st = bt_ctx_create(s->cfg.device, &s->ctx_light);
+    if (st == BT_OK && s->cfg.light_workers == 2) st = bt_ctx_create(s->cfg.device, &s->ctx_light2);
     if (st == BT_OK) st = bt_ctx_create(s->cfg.device, &s->ctx_heavy);
     if (st == BT_OK) st = bt_ctx_create(s->cfg.device, &s->ctx_large);
     if (st == BT_OK) {
@@ -204,15 +215,18 @@ int bt_serve_start(const bt_serve_config* cfg, bt_server** out) {
         // a share of all CUs is a cooperative launch on the whole device
         auto share = [cus](int x) { return x >= cus ? 0 : x; };
         if (st == BT_OK) st = bt_ctx_set_option(s->ctx_light, BT_OPT_PEEL_BLOCKS, share(light));
+        if (st == BT_OK && s->ctx_light2) st = bt_ctx_set_option(s->ctx_light2, BT_OPT_PEEL_BLOCKS, share(light));
         if (st == BT_OK) st = bt_ctx_set_option(s->ctx_heavy, BT_OPT_PEEL_BLOCKS, share(heavy));
         if (st == BT_OK) st = bt_ctx_set_option(s->ctx_large, BT_OPT_PEEL_BLOCKS, share(large));
         // a served batch is one union on its lane's CUs
         if (st == BT_OK) st = bt_ctx_set_option(s->ctx_light, BT_OPT_BATCH_LANES, 1);
+        if (st == BT_OK && s->ctx_light2) st = bt_ctx_set_option(s->ctx_light2, BT_OPT_BATCH_LANES, 1);
         if (st == BT_OK) st = bt_ctx_set_option(s->ctx_heavy, BT_OPT_BATCH_LANES, 1);
         if (st == BT_OK) st = bt_ctx_set_option(s->ctx_large, BT_OPT_BATCH_LANES, 1);
     }
     auto destroy_ctx = [s] {
         if (s->ctx_light) bt_ctx_destroy(s->ctx_light);
+        if (s->ctx_light2) bt_ctx_destroy(s->ctx_light2);
         if (s->ctx_heavy) bt_ctx_destroy(s->ctx_heavy);
         if (s->ctx_large) bt_ctx_destroy(s->ctx_large);
     };
@@ -223,6 +237,7 @@ int bt_serve_start(const bt_serve_config* cfg, bt_server** out) {
     }
     try {
         s->thr_light = std::thread([s] { s->run_batches(s->light_q, s->ctx_light); });
+        if (s->ctx_light2) s->thr_light2 = std::thread([s] { s->run_batches(s->light_q, s->ctx_light2); });
         s->thr_heavy = std::thread([s] { s->run_batches(s->heavy_q, s->ctx_heavy); });
         s->thr_large = std::thread([s] { s->run_batches(s->large_q, s->ctx_large); });
     } catch (...) {
@@ -231,7 +246,7 @@ int bt_serve_start(const bt_serve_config* cfg, bt_server** out) {
             s->stopping = true;
         }
         s->cv_work.notify_all();
-        for (std::thread* t : {&s->thr_light, &s->thr_heavy, &s->thr_large})
+        for (std::thread* t : {&s->thr_light, &s->thr_light2, &s->thr_heavy, &s->thr_large})
             if (t->joinable()) t->join();
         destroy_ctx();
         delete s;
@@ -333,9 +348,10 @@ int bt_serve_stop(bt_server* s) {
         s->stopping = true;
     }
     s->cv_work.notify_all();
-    for (std::thread* t : {&s->thr_light, &s->thr_heavy, &s->thr_large})
+    for (std::thread* t : {&s->thr_light, &s->thr_light2, &s->thr_heavy, &s->thr_large})
         if (t->joinable()) t->join();
     bt_ctx_destroy(s->ctx_light);
+    if (s->ctx_light2) bt_ctx_destroy(s->ctx_light2);
     bt_ctx_destroy(s->ctx_heavy);
     bt_ctx_destroy(s->ctx_large);
     delete s;

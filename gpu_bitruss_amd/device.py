@@ -329,8 +329,9 @@ class Server:
     admission budget."""
 
     def __init__(self, device: int = 0, budget_bytes: int = 0, large_edges: int = 0, max_batch: int = 0,
-                 light_wedges: int = 0, batch_wedges: int = 0, cus: tuple = (0, 0, 0)):
-        cfg = _lib.BtServeConfig(device, budget_bytes, large_edges, max_batch, light_wedges, batch_wedges, *cus)
+                 light_wedges: int = 0, batch_wedges: int = 0, cus: tuple = (0, 0, 0), light_workers: int = 0):
+        cfg = _lib.BtServeConfig(device, budget_bytes, large_edges, max_batch, light_wedges, batch_wedges, *cus,
+                                 light_workers)
         self._h = C.c_void_p()
         _lib.check("bt_serve_start", _lib.lib().bt_serve_start(C.byref(cfg), C.byref(self._h)))
         self._m = {}

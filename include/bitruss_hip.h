@@ -302,7 +302,9 @@ int bt_batch_run(bt_ctx* ctx, bt_batch* b, int algo, uint64_t* const* outs, bt_b
  * batch, while requests that arrive meanwhile wait for the next batch. Small
  * requests with fewer than light_wedges priority wedges go to the light
  * lane, the others to the heavy lane; large requests are batched on a lane
- * of their own. The lanes' persistent peels run on disjoint CU shares
+ * of their own (the light lane has two workers: requests that arrive while
+ * one worker's batch runs are taken by the other at once). The lanes'
+ * persistent peels run on disjoint CU shares
  * (light_cus, heavy_cus, large_cus, each rounded down to a multiple of the
  * 8 XCDs, at least 8), so no lane waits for another's. A request is
  * admitted when the estimated device bytes of the running requests plus its
@@ -325,6 +327,11 @@ typedef struct bt_serve_config {
     uint32_t light_cus;     /* CUs of the light lane's peels (0: 1/4)       */
     uint32_t heavy_cus;     /* CUs of the heavy lane's peels (0: 3/8)       */
     uint32_t large_cus;     /* CUs of the large lane's peels (0: the rest)  */
+    uint32_t light_workers; /* workers (contexts) of the light lane, 1 or 2
+                               (0: 2): while one worker's batch runs, the
+                               other takes the requests that arrive, and
+                               their copy, support count and bloom index
+                               overlap the running peel                     */
 } bt_serve_config;
 
 typedef struct bt_serve_stats {
