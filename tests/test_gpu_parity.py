@@ -408,18 +408,36 @@ def test_skewed_60m_msp_prefix(dev):
 
 
 @pytest.mark.gpu
-def test_skewed_60m_msp_whole(dev):
-    """The 60M-edge graph's whole msp decomposition against the bloom
-    oracle's digest and level count (or_msp_bloom, cpu_bitruss.cpp:536-597),
-    when the oracle's run has been recorded."""
+@pytest.mark.timeout(1000)
+def test_skewed_60m_msp_whole(dev, tmp_path):
+    """The 60M-edge graph's whole msp decomposition through the drop-in
+    command line (`coh --gpu --algo msp --dump`, main.cu:96-129) against the
+    bloom oracle's digest and level count (or_msp_bloom,
+    cpu_bitruss.cpp:536-597). The peel reports each time slice (about every
+    2 s) to a log under gpurun_out/, so the run shows its progress while the
+    test's own output is held by pytest."""
     gold = json.load(open(os.path.join(GOLDEN, "skewed_60m.json")))
     if "msp_sha256" not in gold:
         pytest.skip("the oracle's whole msp of skewed_60m is not recorded (make_golden.py --msp-bloom)")
-    g = gb.Graph.generate("skewed_60m")
-    with gb.Device(dev.device) as d:
-        d.set_msp_path("index")
-        dg = d.upload(g)
-        say("skewed_60m whole msp")
-        res, levels = dg.msp()
-        dg.free()
-    assert digest(res) == gold["msp_sha256"] and levels == gold["msp_levels"]
+    import subprocess
+
+    from gpu_bitruss_amd import _lib
+
+    out_dir = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
+    os.makedirs(out_dir, exist_ok=True)
+    log_path = os.path.join(out_dir, "skewed_60m_msp_progress.log")
+    binf, dump = str(tmp_path / "skewed_60m.bin"), str(tmp_path / "skewed_60m_msp.u64")
+    with open(log_path, "a") as log:
+        log.write("[skewed_60m] generating the graph\n")
+        log.flush()
+        gb.Graph.generate("skewed_60m").graph_to_bin(binf)
+        log.write("[skewed_60m] coh --gpu --algo msp\n")
+        log.flush()
+        env = dict(os.environ, BT_PEEL_PROGRESS="1")
+        r = subprocess.run([_lib.COH_PATH, "--gpu", "--algo", "msp", "--graph", binf, "--dump", dump, "--json"],
+                           stdout=subprocess.PIPE, stderr=log, text=True, env=env, timeout=900)
+    assert r.returncode == 0, r.stdout
+    res = np.fromfile(dump, dtype="<u8")
+    summary = json.loads([l for l in r.stdout.splitlines() if l.startswith("{")][0])
+    assert res.size == gold["m"]
+    assert digest(res) == gold["msp_sha256"] and summary["iters"] == gold["msp_levels"]
