@@ -132,6 +132,7 @@ OPT_PEEL_C2CAP = 9
 OPT_PEEL_WINDOW_EDGES = 10
 OPT_PEEL_BIG_MIDDLES = 11
 OPT_HIDX_GRAPH_ROUNDS = 12
+OPT_HIDX_SORTED = 13
 # statuses (enum bt_status, include/bt_graph.h) the wrappers act on
 BT_PARTIAL = 10
 BT_ERR_BUSY = 11

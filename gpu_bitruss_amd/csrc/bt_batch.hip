@@ -343,6 +343,7 @@ int bt_batch_run(bt_ctx* ctx, bt_batch* b, int algo, uint64_t* const* outs, bt_b
         lc->peel_window_edges = ctx->peel_window_edges;
         lc->peel_big_middles = ctx->peel_big_middles;
         lc->hidx_graph_rounds = ctx->hidx_graph_rounds;
+        lc->hidx_sorted = ctx->hidx_sorted;
         std::copy(ctx->tune, ctx->tune + BT_TUNE_CLASSES, lc->tune);
         lc->peel_blocks = share < lc->cu_count ? share : 0;
     }

@@ -23,6 +23,19 @@
 // prev[e] ~ 10^6, a third above). One wave per edge; the wave walks the
 // concatenation of the edge's blooms 64 middles at a time.
 //
+// Sorted chunks: that pass costs the sizes of the edge's blooms, which on a
+// skewed graph run to tens of thousands of middles per bloom. Each round
+// therefore also keeps, for every bloom above kSmallBloom middles, its mv
+// values (clamped to 32 bits, exact for thresholds below 2^32) sorted in
+// descending chunks of kChunk, re-sorting only the blooms with a change in
+// the previous round. The number of an edge's values >= t is then, per
+// bloom with prev[partner] >= t, a binary search per chunk (a pass for the
+// small blooms) minus the edge's own middle, and the h-index is found by a
+// binary search over t in [0, prev[e]] after a first count at t = prev[e]
+// (most edges keep their value in the late rounds). Per edge the wave
+// estimates both costs from its blooms' sizes and takes the cheaper way;
+// both give the same h-index.
+//
 // Round loop: the reference reads a change flag back after every round
 // (bitruss_hidx.cu:219-231). Here the rounds are captured once into a HIP
 // graph of K rounds (BT_OPT_HIDX_GRAPH_ROUNDS, default 8) whose kernels keep
@@ -50,6 +63,11 @@ namespace {
 constexpr int kHT = 512;              // block size
 constexpr int kHW = kHT / kWave;      // waves per block
 constexpr uint32_t kHB = 1024;        // histogram bins per wave
+constexpr uint32_t kSmallBloom = 32;  // blooms up to this size: counted by a pass over their values
+constexpr uint32_t kChunk = 2048;     // larger blooms: mv values sorted in chunks of this length
+constexpr int kST = 256;              // block size of the chunk sort
+constexpr uint32_t kCacheInc = 256;   // incidences whose bloom data a wave keeps in LDS (in its histogram)
+constexpr uint32_t kSearchCost = 12;  // cost of a binary search in one chunk, in values streamed
 
 // Device-side loop state (control words of the graph): round counter,
 // change flag of the current round, done flag.
@@ -88,13 +106,79 @@ __global__ void k_hidx_clear(uint8_t* __restrict__ p, size_t n, const unsigned l
         p[i] = 0;
 }
 
+// mv of every record, and its 32-bit copy smv (null: none) for the sorted
+// chunks. A bloom without a change in the previous round (dirty_in, null:
+// every bloom) keeps the values it has; in graph rounds round 1 ignores
+// dirty_in.
 __global__ void k_hidx_mv(uint32_t R, const uint64_t* __restrict__ rec, const uint64_t* __restrict__ prev,
-                          uint64_t* __restrict__ mv, const unsigned long long* done) {
+                          uint64_t* __restrict__ mv, uint32_t* __restrict__ smv, const uint32_t* __restrict__ bid,
+                          const uint8_t* __restrict__ dirty_in, const unsigned long long* dev_round,
+                          const unsigned long long* done) {
     if (done && *done) return;
+    if (dev_round && *dev_round == 1) dirty_in = nullptr;
     for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < R; p += uint64_t(gridDim.x) * blockDim.x) {
+        if (dirty_in && !dirty_in[bid[p] - 1]) continue;
         uint64_t const r = rec[p];
         uint64_t const a = prev[uint32_t(r >> 32)], b = prev[uint32_t(r)];
-        mv[p] = a < b ? a : b;
+        uint64_t const v = a < b ? a : b;
+        mv[p] = v;
+        if (smv) smv[p] = v < 0xFFFFFFFFull ? uint32_t(v) : 0xFFFFFFFFu;
+    }
+}
+
+// The chunks of the blooms above kSmallBloom middles, as bloom << 32 |
+// chunk: counted into *cnt (list null), then listed at *cnt's positions
+// (the order is free: each chunk is sorted on its own).
+__global__ void k_hidx_chunks(uint32_t NB, const uint32_t* __restrict__ bstart, unsigned long long* cnt,
+                              uint64_t* __restrict__ list) {
+    for (size_t b = blockIdx.x * size_t(blockDim.x) + threadIdx.x; b < NB; b += size_t(gridDim.x) * blockDim.x) {
+        uint32_t const k = bstart[b + 1] - bstart[b];
+        if (k <= kSmallBloom) continue;
+        uint32_t const nch = (k + kChunk - 1) / kChunk;
+        unsigned long long const at = atomicAdd(cnt, (unsigned long long)nch);
+        if (list)
+            for (uint32_t c = 0; c < nch; c++) list[at + c] = (uint64_t(b) << 32) | c;
+    }
+}
+
+// Sorts each listed chunk of smv in descending order (a bitonic sort in
+// LDS, padded with zeros to a power of two), skipping the blooms without
+// a change in the previous round. One workgroup per chunk.
+__global__ __launch_bounds__(kST) void k_hidx_sort(const uint64_t* __restrict__ list, uint64_t n,
+                                                   const uint32_t* __restrict__ bstart, uint32_t* __restrict__ smv,
+                                                   const uint8_t* __restrict__ dirty_in,
+                                                   const unsigned long long* dev_round,
+                                                   const unsigned long long* done) {
+    if (done && *done) return;
+    if (dev_round && *dev_round == 1) dirty_in = nullptr;
+    __shared__ uint32_t s[kChunk];
+    for (uint64_t i = blockIdx.x; i < n; i += gridDim.x) {
+        uint64_t const it = list[i];
+        uint32_t const b = uint32_t(it >> 32), ch = uint32_t(it);
+        if (dirty_in && !dirty_in[b]) continue;  // the same for the whole workgroup
+        uint32_t const b0 = bstart[b], k = bstart[b + 1] - b0;
+        uint32_t const start = b0 + ch * kChunk;
+        uint32_t const len = std::min(kChunk, k - ch * kChunk);
+        uint32_t P = 64;
+        while (P < len) P <<= 1;
+        __syncthreads();  // the previous chunk's values are written out
+        for (uint32_t j = threadIdx.x; j < P; j += kST) s[j] = j < len ? smv[start + j] : 0u;
+        for (uint32_t size = 2; size <= P; size <<= 1) {
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                __syncthreads();
+                for (uint32_t j = threadIdx.x; j < P / 2; j += kST) {
+                    uint32_t const lo = 2 * j - (j & (stride - 1)), hi = lo + stride;
+                    bool const desc = (lo & size) == 0;  // the last merge (size P) is descending throughout
+                    uint32_t const x = s[lo], y = s[hi];
+                    if ((x < y) == desc) {
+                        s[lo] = y;
+                        s[hi] = x;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < len; j += kST) smv[start + j] = s[j];
     }
 }
 
@@ -108,6 +192,9 @@ struct HidxArgs {
     const uint32_t* bid;     // per record: bloom index + 1
     const uint32_t* bstart;  // per bloom: first record, then R
     const uint64_t* mv;      // per record: min(prev) of its two edges
+    // per record: mv clamped to 32 bits, in descending chunks of kChunk for
+    // the blooms above kSmallBloom middles (null: every edge streams)
+    const uint32_t* smv;
     unsigned long long* changed;
     const uint32_t* egraph;  // batches: graph of each edge (else null)
     uint32_t* glast;         // batches: per graph, last round with a change
@@ -190,6 +277,121 @@ __device__ __forceinline__ void bin_values(const HidxArgs& a, uint32_t e, uint64
     }
 }
 
+// Bloom data of edge e's incidence q: its partner's value clamped to 32 bits
+// (0: the bloom gives e no non-zero value), the bloom's first record and
+// its size (not read for a zero partner).
+struct IncInfo {
+    uint32_t pp, base, k;
+};
+
+__device__ __forceinline__ IncInfo inc_info(const HidxArgs& a, uint32_t e, uint64_t q) {
+    uint32_t const p = a.inc[q];
+    uint64_t const r = a.rec[p];
+    uint32_t const partner = uint32_t(r >> 32) == e ? uint32_t(r) : uint32_t(r >> 32);
+    uint64_t const pp = a.prev[partner];
+    IncInfo x{pp < 0xFFFFFFFFull ? uint32_t(pp) : 0xFFFFFFFFu, 0u, 0u};
+    if (x.pp) {
+        uint32_t const b = a.bid[p] - 1;
+        x.base = a.bstart[b];
+        x.k = a.bstart[b + 1] - x.base;
+    }
+    return x;
+}
+
+// Values >= t among the other middles of one bloom, for an edge whose
+// partner and own value are >= t >= 1: the bloom's mv values >= t (a pass
+// over a small bloom, a binary search per descending chunk of a larger
+// one) less the edge's own middle, whose mv = min(prev[e], partner) >= t.
+__device__ __forceinline__ uint32_t bloom_count(const uint32_t* __restrict__ smv, uint32_t base, uint32_t k,
+                                                uint32_t t) {
+    uint32_t c = 0;
+    if (k <= kSmallBloom) {
+        for (uint32_t j = 0; j < k; j++) c += smv[base + j] >= t ? 1u : 0u;
+    } else {
+        for (uint32_t c0 = 0; c0 < k; c0 += kChunk) {
+            const uint32_t* s = smv + base + c0;
+            uint32_t lo = 0, hi = std::min(kChunk, k - c0);
+            while (lo < hi) {  // the first position below t
+                uint32_t const mid = (lo + hi) >> 1;
+                if (s[mid] >= t)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            c += lo;
+        }
+    }
+    return c - 1;
+}
+
+// Number of edge e's butterfly values >= t (1 <= t <= prev[e]), summed over
+// the wave; the bloom data of the first kCacheInc incidences comes from
+// the wave's LDS cache (pp, base and k in three rows of kCacheInc).
+__device__ uint64_t count_at_least(const HidxArgs& a, uint32_t e, uint64_t q0, uint64_t q1, uint32_t t,
+                                   const uint32_t* cache) {
+    int const lane = threadIdx.x & (kWave - 1);
+    uint64_t c = 0;
+    for (uint64_t b0 = q0; b0 < q1; b0 += kWave) {
+        uint64_t const q = b0 + lane;
+        if (q < q1) {
+            uint64_t const i = q - q0;
+            IncInfo const x = i < kCacheInc ? IncInfo{cache[i], cache[kCacheInc + i], cache[2 * kCacheInc + i]}
+                                            : inc_info(a, e, q);
+            if (x.pp >= t) c += bloom_count(a.smv, x.base, x.k, t);
+        }
+    }
+    return wave_sum_u64(c);
+}
+
+// The sorted-chunk way for edge e (whole wave): fills the cache, estimates
+// the cost of a count against the values one stream pass reads, counts at
+// t = prev[e] and, while it stays the cheaper way, searches the h-index.
+// Returns false (and leaves *h) when the stream passes should do it.
+__device__ bool hidx_sorted(const HidxArgs& a, uint32_t e, uint64_t q0, uint64_t q1, uint32_t pe, uint32_t* cache,
+                            uint64_t* h) {
+    int const lane = threadIdx.x & (kWave - 1);
+    uint64_t cost = 0, vals = 0;
+    for (uint64_t b0 = q0; b0 < q1; b0 += kWave) {
+        uint64_t const q = b0 + lane;
+        if (q < q1) {
+            IncInfo const x = inc_info(a, e, q);
+            uint64_t const i = q - q0;
+            if (i < kCacheInc) {
+                cache[i] = x.pp;
+                cache[kCacheInc + i] = x.base;
+                cache[2 * kCacheInc + i] = x.k;
+            }
+            if (x.pp) {
+                vals += x.k - 1;
+                cost += x.k <= kSmallBloom ? x.k : uint64_t((x.k + kChunk - 1) / kChunk) * kSearchCost;
+            }
+        }
+    }
+    cost = wave_sum_u64(cost);
+    vals = wave_sum_u64(vals);
+    __builtin_amdgcn_wave_barrier();
+    // the stream passes: one up to kHB - 1, two up to kHB^2 - 1, else three
+    uint64_t const passes = pe < kHB ? 1 : (uint64_t(pe) < uint64_t(kHB) * kHB ? 2 : 3);
+    if (cost >= vals * passes) return false;
+    if (count_at_least(a, e, q0, q1, pe, cache) >= pe) {
+        *h = pe;
+        return true;
+    }
+    // binary search over [0, pe - 1]: about log2(pe) more counts
+    uint64_t const steps = 32 - __builtin_clz(pe);
+    if (steps * cost >= vals * passes) return false;
+    uint32_t lo = 0, hi = pe - 1;  // C(lo) >= lo holds for lo = 0
+    while (lo < hi) {
+        uint32_t const mid = lo + (hi - lo + 1) / 2;
+        if (count_at_least(a, e, q0, q1, mid, cache) >= mid)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    *h = lo;
+    return true;
+}
+
 __global__ __launch_bounds__(kHT) void k_hidx_round(HidxArgs a) {
     if (a.done) {
         if (*a.done) return;
@@ -217,11 +419,12 @@ __global__ __launch_bounds__(kHT) void k_hidx_round(HidxArgs a) {
             if (lane == 0) a.curr[e] = pe;
             continue;
         }
+        uint64_t h = 0;
+        bool const sorted = a.smv && pe <= 0xFFFFFFFFull && hidx_sorted(a, e, q0, q1, uint32_t(pe), hist, &h);
         // candidates [lo, hi]; `above` values lie above hi
         uint64_t lo = 0, hi = pe, above = 0;
         uint64_t width = (pe + kHB) / kHB;  // ceil((pe + 1) / kHB)
-        uint64_t h = 0;
-        for (;;) {
+        for (; !sorted;) {
             for (uint32_t i = lane; i < kHB; i += kWave) hist[i] = 0;
             __builtin_amdgcn_wave_barrier();
             bin_values(a, e, q0, q1, lo, hi, width, pe, hist, s_incl[wv], s_base[wv], s_skip[wv], s_pp[wv]);
@@ -284,10 +487,52 @@ __global__ __launch_bounds__(kHT) void k_hidx_round(HidxArgs a) {
 
 }  // namespace
 
+// The sorted chunks of a call: smv (null: every edge streams its values)
+// and the list of chunks to sort each round.
+struct SortedChunks {
+    uint32_t* smv = nullptr;
+    const uint64_t* list = nullptr;
+    uint64_t n = 0;
+    unsigned grid = 1;
+};
+
+// smv lives in the peel's claim stamps (bclaim, at least R u32) and the
+// chunk list in its claimed-bloom lists (touched): the index build sizes
+// both for every call and h-index rounds use neither otherwise.
+static int hidx_sorted_setup(bt_dgraph* dg, SortedChunks* sc) {
+    bt_ctx* c = dg->ctx;
+    hipStream_t s = c->stream;
+    bt_index& ix = c->ix;
+    *sc = SortedChunks{};
+    if (!c->hidx_sorted || ix.R == 0 || ix.NB == 0 || ix.bclaim.bytes < sizeof(uint32_t) * ix.R) return BT_OK;
+    uint32_t const NB = uint32_t(ix.NB);
+    auto* cnt = reinterpret_cast<unsigned long long*>(dg->ctl + 19);
+    auto const* bstart = static_cast<const uint32_t*>(ix.bstart.ptr);
+    unsigned const grid = grid_for(NB, 256, unsigned(c->cu_count) * 16);
+    BT_HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(uint64_t), s));
+    k_hidx_chunks<<<grid, 256, 0, s>>>(NB, bstart, cnt, nullptr);
+    BT_HIP_TRY(hipGetLastError());
+    uint64_t* h = static_cast<uint64_t*>(c->h_pinned);
+    BT_HIP_TRY(hipMemcpyAsync(h, cnt, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    BT_HIP_TRY(hipStreamSynchronize(s));
+    uint64_t const n = h[0];
+    sc->smv = static_cast<uint32_t*>(ix.bclaim.ptr);
+    if (n) {
+        BT_TRY(bt_buf_reserve(ix.touched, sizeof(uint64_t) * n));
+        BT_HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(uint64_t), s));
+        k_hidx_chunks<<<grid, 256, 0, s>>>(NB, bstart, cnt, static_cast<uint64_t*>(ix.touched.ptr));
+        BT_HIP_TRY(hipGetLastError());
+        sc->list = static_cast<const uint64_t*>(ix.touched.ptr);
+        sc->n = n;
+        sc->grid = unsigned(std::min<uint64_t>(n, uint64_t(c->cu_count) * 16));
+    }
+    return BT_OK;
+}
+
 // The rounds as a captured graph of K rounds (K even), launched until a
 // round without a change; *rounds receives the rounds counted on the device.
-static int hidx_graph_loop(bt_dgraph* dg, uint64_t* mv, unsigned long long* changed, uint8_t* const dirty[2],
-                           size_t nbm, unsigned grid_mv, unsigned grid_e, uint32_t* rounds) {
+static int hidx_graph_loop(bt_dgraph* dg, uint64_t* mv, const SortedChunks& sc, unsigned long long* changed,
+                           uint8_t* const dirty[2], size_t nbm, unsigned grid_mv, unsigned grid_e, uint32_t* rounds) {
     bt_ctx* c = dg->ctx;
     hipStream_t s = c->stream;
     bt_index& ix = c->ix;
@@ -309,7 +554,12 @@ static int hidx_graph_loop(bt_dgraph* dg, uint64_t* mv, unsigned long long* chan
         uint8_t* in = dirty[k & 1];         // and reads dirty[(r + 1) & 1]
         k_hidx_round_begin<<<1, 64, 0, s>>>(st);
         k_hidx_clear<<<grid_clear, 256, 0, s>>>(out, nbm, st.done);
-        if (R) k_hidx_mv<<<grid_mv, 256, 0, s>>>(R, static_cast<const uint64_t*>(ix.vals1.ptr), prev, mv, st.done);
+        if (R)
+            k_hidx_mv<<<grid_mv, 256, 0, s>>>(R, static_cast<const uint64_t*>(ix.vals1.ptr), prev, mv, sc.smv,
+                                              static_cast<const uint32_t*>(ix.bid.ptr), in, st.round, st.done);
+        if (sc.n)
+            k_hidx_sort<<<sc.grid, kST, 0, s>>>(sc.list, sc.n, static_cast<const uint32_t*>(ix.bstart.ptr), sc.smv,
+                                                 in, st.round, st.done);
         HidxArgs a{m,
                    prev,
                    curr,
@@ -319,6 +569,7 @@ static int hidx_graph_loop(bt_dgraph* dg, uint64_t* mv, unsigned long long* chan
                    static_cast<const uint32_t*>(ix.bid.ptr),
                    static_cast<const uint32_t*>(ix.bstart.ptr),
                    mv,
+                   sc.smv,
                    changed,
                    dg->egraph,
                    dg->graph_last,
@@ -391,17 +642,25 @@ int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds_out) {
         size_t const nbm = std::max<size_t>(ix.NB, 1);
         BT_TRY(bt_buf_reserve(ix.hdirty, 2 * nbm));
         uint8_t* dirty[2] = {static_cast<uint8_t*>(ix.hdirty.ptr), static_cast<uint8_t*>(ix.hdirty.ptr) + nbm};
+        SortedChunks sc;
+        BT_TRY(hidx_sorted_setup(dg, &sc));
         bool const progress = getenv("BT_PEEL_PROGRESS") != nullptr;
         auto const t_first = std::chrono::steady_clock::now();
         if (c->hidx_graph_rounds > 0) {
-            BT_TRY(hidx_graph_loop(dg, mv, changed, dirty, nbm, grid_mv, grid_e, &rounds));
+            BT_TRY(hidx_graph_loop(dg, mv, sc, changed, dirty, nbm, grid_mv, grid_e, &rounds));
             prev = dg->res;  // the last round left both buffers equal
         }
         while (c->hidx_graph_rounds == 0) {
             rounds++;
             BT_HIP_TRY(hipMemsetAsync(changed, 0, sizeof(uint64_t), s));
             BT_HIP_TRY(hipMemsetAsync(dirty[rounds & 1], 0, nbm, s));
-            if (R) k_hidx_mv<<<grid_mv, 256, 0, s>>>(R, static_cast<const uint64_t*>(ix.vals1.ptr), prev, mv, nullptr);
+            uint8_t const* in = rounds > 1 ? dirty[(rounds + 1) & 1] : nullptr;
+            if (R)
+                k_hidx_mv<<<grid_mv, 256, 0, s>>>(R, static_cast<const uint64_t*>(ix.vals1.ptr), prev, mv, sc.smv,
+                                                  static_cast<const uint32_t*>(ix.bid.ptr), in, nullptr, nullptr);
+            if (sc.n)
+                k_hidx_sort<<<sc.grid, kST, 0, s>>>(sc.list, sc.n, static_cast<const uint32_t*>(ix.bstart.ptr),
+                                                     sc.smv, in, nullptr, nullptr);
             HidxArgs a{m,
                        prev,
                        curr,
@@ -411,11 +670,12 @@ int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds_out) {
                        static_cast<const uint32_t*>(ix.bid.ptr),
                        static_cast<const uint32_t*>(ix.bstart.ptr),
                        mv,
+                       sc.smv,
                        changed,
                        dg->egraph,
                        dg->graph_last,
                        rounds,
-                       rounds > 1 ? dirty[(rounds + 1) & 1] : nullptr,
+                       in,
                        dirty[rounds & 1],
                        nullptr,
                        nullptr};

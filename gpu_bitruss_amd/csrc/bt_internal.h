@@ -113,6 +113,7 @@ struct bt_ctx {
     uint64_t peel_window_edges = uint64_t(4) << 20;  // live edges from which the collect uses a window
     int peel_big_middles = 1024;   // blooms beyond this are walked by a whole workgroup
     int hidx_graph_rounds = 8;     // h-index rounds per captured graph (0: one by one)
+    int hidx_sorted = 1;           // h-index counts over sorted chunks of the larger blooms (0: streams only)
     // counting launch settings per graph-size class (bt_tune_*), and the
     // settings an autotuning run imposes on the calls it times (null: the
     // table decides)

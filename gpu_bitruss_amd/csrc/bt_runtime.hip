@@ -338,6 +338,10 @@ int bt_ctx_set_option(bt_ctx* ctx, int option, int64_t value) {
             if (value < 0 || value > 1024) return BT_ERR_INVALID;
             ctx->hidx_graph_rounds = int(value);
             return BT_OK;
+        case BT_OPT_HIDX_SORTED:
+            if (value < 0 || value > 1) return BT_ERR_INVALID;
+            ctx->hidx_sorted = int(value);
+            return BT_OK;
         case BT_OPT_PEEL_BIG_MIDDLES:
             if (value < 1 || value > 0x7FFFFFFF) return BT_ERR_INVALID;
             ctx->peel_big_middles = int(value);

@@ -125,6 +125,12 @@ class Device:
         read-back after each round)."""
         _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(self.handle, _lib.OPT_HIDX_GRAPH_ROUNDS, rounds))
 
+    def set_hidx_sorted(self, on: bool) -> None:
+        """Whether h-index rounds count over sorted chunks of the larger
+        blooms where that is cheaper (the default) or every edge streams its
+        values. Results and round counts are the same."""
+        _lib.check("bt_ctx_set_option", _lib.lib().bt_ctx_set_option(self.handle, _lib.OPT_HIDX_SORTED, int(bool(on))))
+
     def tune_get(self, size_class: int) -> tuple:
         """Counting settings of a graph-size class: (wave_wedges,
         block_wedges, block_threads, hub_threads)."""

@@ -141,11 +141,17 @@ int bt_ctx_trim(bt_ctx* ctx);
  *                          one with a read-back after each, as the reference
  *                          (bitruss_hidx.cu:219-231)
  *   BT_OPT_PEEL_BIG_MIDDLES  blooms with more middles are walked by a whole
- *                          workgroup (default 1,024). Results are the same. */
+ *                          workgroup (default 1,024). Results are the same.
+ *   BT_OPT_HIDX_SORTED     1 (default): each h-index round keeps the values
+ *                          of the blooms above 32 middles in sorted chunks,
+ *                          so that an edge counts its values >= x by binary
+ *                          searches where that is cheaper than a pass; 0:
+ *                          every edge streams its values. Results and round
+ *                          counts are the same. */
 enum { BT_OPT_MSP_PATH = 1, BT_OPT_INDEX_MEM_PCT = 2, BT_OPT_BATCH_WEDGES = 3, BT_OPT_BATCH_LEVELS = 4,
        BT_OPT_PEEL_BLOCKS = 5, BT_OPT_PEEL_SLICE_MS = 6, BT_OPT_BATCH_LANES = 7, BT_OPT_PEEL_MAX_LEVEL = 8,
        BT_OPT_PEEL_C2CAP = 9, BT_OPT_PEEL_WINDOW_EDGES = 10, BT_OPT_PEEL_BIG_MIDDLES = 11,
-       BT_OPT_HIDX_GRAPH_ROUNDS = 12 };
+       BT_OPT_HIDX_GRAPH_ROUNDS = 12, BT_OPT_HIDX_SORTED = 13 };
 enum { BT_MSP_PATH_AUTO = 0, BT_MSP_PATH_MERGE = 1, BT_MSP_PATH_INDEX = 2 };
 enum { BT_BATCH_LEVELS_PER_GRAPH = 0, BT_BATCH_LEVELS_UNION = 1 };
 int bt_ctx_set_option(bt_ctx* ctx, int option, int64_t value);
