@@ -298,49 +298,77 @@ __device__ __forceinline__ IncInfo inc_info(const HidxArgs& a, uint32_t e, uint6
     return x;
 }
 
-// Values >= t among the other middles of one bloom, for an edge whose
-// partner and own value are >= t >= 1: the bloom's mv values >= t (a pass
-// over a small bloom, a binary search per descending chunk of a larger
-// one) less the edge's own middle, whose mv = min(prev[e], partner) >= t.
-__device__ __forceinline__ uint32_t bloom_count(const uint32_t* __restrict__ smv, uint32_t base, uint32_t k,
-                                                uint32_t t) {
-    uint32_t c = 0;
+// mv values >= t in one item of a bloom: the whole bloom when it is small
+// (a pass), else its chunk ch (a binary search in descending order).
+__device__ __forceinline__ uint32_t item_count(const uint32_t* __restrict__ smv, uint32_t base, uint32_t k,
+                                               uint32_t ch, uint32_t t) {
     if (k <= kSmallBloom) {
+        uint32_t c = 0;
         for (uint32_t j = 0; j < k; j++) c += smv[base + j] >= t ? 1u : 0u;
-    } else {
-        for (uint32_t c0 = 0; c0 < k; c0 += kChunk) {
-            const uint32_t* s = smv + base + c0;
-            uint32_t lo = 0, hi = std::min(kChunk, k - c0);
-            while (lo < hi) {  // the first position below t
-                uint32_t const mid = (lo + hi) >> 1;
-                if (s[mid] >= t)
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
-            c += lo;
-        }
+        return c;
     }
-    return c - 1;
+    const uint32_t* s = smv + base + ch * kChunk;
+    uint32_t lo = 0, hi = std::min(kChunk, k - ch * kChunk);
+    while (lo < hi) {  // the first position below t
+        uint32_t const mid = (lo + hi) >> 1;
+        if (s[mid] >= t)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
 }
 
 // Number of edge e's butterfly values >= t (1 <= t <= prev[e]), summed over
-// the wave; the bloom data of the first kCacheInc incidences comes from
-// the wave's LDS cache (pp, base and k in three rows of kCacheInc).
+// the wave. Each bloom with prev[partner] >= t gives its mv values >= t less
+// the edge's own middle (mv = min(prev[e], partner) >= t). The wave takes
+// 64 incidences at a time (the bloom data of the first kCacheInc from its
+// LDS cache: pp, base and k in three rows of kCacheInc) and spreads their
+// items (a small bloom, or one chunk of a larger one) over its lanes, so
+// that the chunks of one large bloom are searched side by side. s_incl,
+// s_base and s_k are the wave's 64-entry LDS rows.
 __device__ uint64_t count_at_least(const HidxArgs& a, uint32_t e, uint64_t q0, uint64_t q1, uint32_t t,
-                                   const uint32_t* cache) {
+                                   const uint32_t* cache, uint32_t* s_incl, uint32_t* s_base, uint32_t* s_k) {
     int const lane = threadIdx.x & (kWave - 1);
-    uint64_t c = 0;
+    uint64_t c = 0, own = 0;
     for (uint64_t b0 = q0; b0 < q1; b0 += kWave) {
         uint64_t const q = b0 + lane;
+        IncInfo x{0u, 0u, 0u};
         if (q < q1) {
             uint64_t const i = q - q0;
-            IncInfo const x = i < kCacheInc ? IncInfo{cache[i], cache[kCacheInc + i], cache[2 * kCacheInc + i]}
-                                            : inc_info(a, e, q);
-            if (x.pp >= t) c += bloom_count(a.smv, x.base, x.k, t);
+            x = i < kCacheInc ? IncInfo{cache[i], cache[kCacheInc + i], cache[2 * kCacheInc + i]}
+                              : inc_info(a, e, q);
         }
+        uint32_t const items = x.pp >= t ? (x.k <= kSmallBloom ? 1u : (x.k + kChunk - 1) / kChunk) : 0u;
+        own += __builtin_popcountll(__ballot(items != 0));
+        if (!__ballot(items > 1)) {  // one item per bloom: each lane its own
+            if (items) c += item_count(a.smv, x.base, x.k, 0, t);
+            continue;
+        }
+        uint32_t const incl = wave_incl_scan(items);
+        uint32_t const total = __shfl(incl, kWave - 1, kWave);
+        __builtin_amdgcn_wave_barrier();
+        s_incl[lane] = incl;
+        s_base[lane] = x.base;
+        s_k[lane] = x.k;
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t t0 = 0; t0 < total; t0 += kWave) {
+            uint32_t const pos = t0 + lane;
+            if (pos < total) {
+                // the incidence of item pos: first prefix above pos
+                uint32_t j = 0;
+#pragma unroll
+                for (uint32_t step = kWave / 2; step > 0; step >>= 1) {
+                    uint32_t const k = j + step;
+                    j = s_incl[k - 1] <= pos ? k : j;
+                }
+                uint32_t const first = j ? s_incl[j - 1] : 0u;
+                c += item_count(a.smv, s_base[j], s_k[j], pos - first, t);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
     }
-    return wave_sum_u64(c);
+    return wave_sum_u64(c) - own;
 }
 
 // The sorted-chunk way for edge e (whole wave): fills the cache, estimates
@@ -348,7 +376,7 @@ __device__ uint64_t count_at_least(const HidxArgs& a, uint32_t e, uint64_t q0, u
 // t = prev[e] and, while it stays the cheaper way, searches the h-index.
 // Returns false (and leaves *h) when the stream passes should do it.
 __device__ bool hidx_sorted(const HidxArgs& a, uint32_t e, uint64_t q0, uint64_t q1, uint32_t pe, uint32_t* cache,
-                            uint64_t* h) {
+                            uint32_t* s_incl, uint32_t* s_base, uint32_t* s_k, uint64_t* h) {
     int const lane = threadIdx.x & (kWave - 1);
     uint64_t cost = 0, vals = 0;
     for (uint64_t b0 = q0; b0 < q1; b0 += kWave) {
@@ -373,7 +401,7 @@ __device__ bool hidx_sorted(const HidxArgs& a, uint32_t e, uint64_t q0, uint64_t
     // the stream passes: one up to kHB - 1, two up to kHB^2 - 1, else three
     uint64_t const passes = pe < kHB ? 1 : (uint64_t(pe) < uint64_t(kHB) * kHB ? 2 : 3);
     if (cost >= vals * passes) return false;
-    if (count_at_least(a, e, q0, q1, pe, cache) >= pe) {
+    if (count_at_least(a, e, q0, q1, pe, cache, s_incl, s_base, s_k) >= pe) {
         *h = pe;
         return true;
     }
@@ -383,7 +411,7 @@ __device__ bool hidx_sorted(const HidxArgs& a, uint32_t e, uint64_t q0, uint64_t
     uint32_t lo = 0, hi = pe - 1;  // C(lo) >= lo holds for lo = 0
     while (lo < hi) {
         uint32_t const mid = lo + (hi - lo + 1) / 2;
-        if (count_at_least(a, e, q0, q1, mid, cache) >= mid)
+        if (count_at_least(a, e, q0, q1, mid, cache, s_incl, s_base, s_k) >= mid)
             lo = mid;
         else
             hi = mid - 1;
@@ -420,7 +448,8 @@ __global__ __launch_bounds__(kHT) void k_hidx_round(HidxArgs a) {
             continue;
         }
         uint64_t h = 0;
-        bool const sorted = a.smv && pe <= 0xFFFFFFFFull && hidx_sorted(a, e, q0, q1, uint32_t(pe), hist, &h);
+        bool const sorted = a.smv && pe <= 0xFFFFFFFFull &&
+                            hidx_sorted(a, e, q0, q1, uint32_t(pe), hist, s_incl[wv], s_base[wv], s_skip[wv], &h);
         // candidates [lo, hi]; `above` values lie above hi
         uint64_t lo = 0, hi = pe, above = 0;
         uint64_t width = (pe + kHB) / kHB;  // ceil((pe + 1) / kHB)
