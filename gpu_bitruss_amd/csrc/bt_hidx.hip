@@ -401,20 +401,28 @@ __device__ bool hidx_sorted(const HidxArgs& a, uint32_t e, uint64_t q0, uint64_t
     // the stream passes: one up to kHB - 1, two up to kHB^2 - 1, else three
     uint64_t const passes = pe < kHB ? 1 : (uint64_t(pe) < uint64_t(kHB) * kHB ? 2 : 3);
     if (cost >= vals * passes) return false;
-    if (count_at_least(a, e, q0, q1, pe, cache, s_incl, s_base, s_k) >= pe) {
+    uint64_t const c0 = count_at_least(a, e, q0, q1, pe, cache, s_incl, s_base, s_k);
+    if (c0 >= pe) {
         *h = pe;
         return true;
     }
-    // binary search over [0, pe - 1]: about log2(pe) more counts
-    uint64_t const steps = 32 - __builtin_clz(pe);
+    // The count C(t) does not grow with t, so every count bounds the answer
+    // from both sides: C(x) < x rules out t >= x and admits t <= C(x)
+    // (C(t) >= C(x) >= t there); C(x) >= x admits x and rules out
+    // t > C(x). The search starts from [C(pe), pe - 1].
+    uint32_t lo = uint32_t(c0), hi = pe - 1;
+    uint64_t const steps = 32 - __builtin_clz(hi - lo + 1);
     if (steps * cost >= vals * passes) return false;
-    uint32_t lo = 0, hi = pe - 1;  // C(lo) >= lo holds for lo = 0
     while (lo < hi) {
         uint32_t const mid = lo + (hi - lo + 1) / 2;
-        if (count_at_least(a, e, q0, q1, mid, cache, s_incl, s_base, s_k) >= mid)
+        uint64_t const cm = count_at_least(a, e, q0, q1, mid, cache, s_incl, s_base, s_k);
+        if (cm >= mid) {
             lo = mid;
-        else
+            hi = cm < hi ? uint32_t(cm) : hi;
+        } else {
             hi = mid - 1;
+            lo = cm > lo ? uint32_t(cm) : lo;
+        }
     }
     *h = lo;
     return true;
