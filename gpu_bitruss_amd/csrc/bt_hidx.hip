@@ -31,8 +31,11 @@
 // the previous round. The number of an edge's values >= t is then, per
 // bloom with prev[partner] >= t, a binary search per chunk (a pass for the
 // small blooms) minus the edge's own middle, and the h-index is found by a
-// binary search over t in [0, prev[e]] after a first count at t = prev[e]
-// (most edges keep their value in the late rounds). Per edge the wave
+// search over t after a first count at t = prev[e] (most edges keep their
+// value in the late rounds), each count narrowing the range from both
+// sides. An edge none of whose blooms changed in the previous round keeps
+// its value; the marks come per edge from the changed blooms' records.
+// Per edge the wave
 // estimates both costs from its blooms' sizes and takes the cheaper way;
 // both give the same h-index.
 //
@@ -108,17 +111,23 @@ __global__ void k_hidx_clear(uint8_t* __restrict__ p, size_t n, const unsigned l
 
 // mv of every record, and its 32-bit copy smv (null: none) for the sorted
 // chunks. A bloom without a change in the previous round (dirty_in, null:
-// every bloom) keeps the values it has; in graph rounds round 1 ignores
-// dirty_in.
+// every bloom) keeps the values it has; the two edges of each record of a
+// bloom with a change get their mark in emark, which tells the round which
+// edges to evaluate again (an edge lies in a bloom exactly when it is an
+// edge of one of its records). In graph rounds round 1 ignores dirty_in.
 __global__ void k_hidx_mv(uint32_t R, const uint64_t* __restrict__ rec, const uint64_t* __restrict__ prev,
                           uint64_t* __restrict__ mv, uint32_t* __restrict__ smv, const uint32_t* __restrict__ bid,
-                          const uint8_t* __restrict__ dirty_in, const unsigned long long* dev_round,
-                          const unsigned long long* done) {
+                          const uint8_t* __restrict__ dirty_in, uint8_t* __restrict__ emark,
+                          const unsigned long long* dev_round, const unsigned long long* done) {
     if (done && *done) return;
     if (dev_round && *dev_round == 1) dirty_in = nullptr;
     for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < R; p += uint64_t(gridDim.x) * blockDim.x) {
         if (dirty_in && !dirty_in[bid[p] - 1]) continue;
         uint64_t const r = rec[p];
+        if (dirty_in) {
+            emark[uint32_t(r >> 32)] = 1;
+            emark[uint32_t(r)] = 1;
+        }
         uint64_t const a = prev[uint32_t(r >> 32)], b = prev[uint32_t(r)];
         uint64_t const v = a < b ? a : b;
         mv[p] = v;
@@ -203,23 +212,14 @@ struct HidxArgs {
     // (null: every bloom, the first round), and the marks of this round
     const uint8_t* dirty_in;
     uint8_t* dirty_out;
+    // per edge: one of its blooms changed in the previous round (set by
+    // k_hidx_mv, cleared here when the edge is evaluated)
+    uint8_t* emark;
     // graph rounds: the device loop state (null: round and dirty_in above
     // hold); the round number is read from it, and round 1 ignores dirty_in
     const unsigned long long* dev_round;
     const unsigned long long* done;
 };
-
-// Whether any bloom holding edge e (records q0..q1 of its incidence list)
-// is marked in dirty (whole wave).
-__device__ __forceinline__ bool any_dirty(const HidxArgs& a, const uint8_t* dirty, uint64_t q0, uint64_t q1) {
-    int const lane = threadIdx.x & (kWave - 1);
-    for (uint64_t b0 = q0; b0 < q1; b0 += kWave) {
-        uint64_t const q = b0 + lane;
-        bool const d = q < q1 && dirty[a.bid[a.inc[q]] - 1];
-        if (__ballot(d)) return true;
-    }
-    return false;
-}
 
 // Streams the values of edge e's butterflies (one wave), clips them to cap
 // (= prev[e], which bounds the answer) and bins those in [lo, hi] into
@@ -451,9 +451,13 @@ __global__ __launch_bounds__(kHT) void k_hidx_round(HidxArgs a) {
         uint64_t const q0 = a.inc_off[e], q1 = a.inc_off[e + 1];
         // no bloom of e saw a change in the previous round: the values of
         // e's butterflies, and so its h-index, are those of that round
-        if (a.dirty_in && !any_dirty(a, a.dirty_in, q0, q1)) {
-            if (lane == 0) a.curr[e] = pe;
-            continue;
+        if (a.dirty_in) {
+            if (!a.emark[e]) {
+                if (lane == 0) a.curr[e] = pe;
+                continue;
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0) a.emark[e] = 0;
         }
         uint64_t h = 0;
         bool const sorted = a.smv && pe <= 0xFFFFFFFFull &&
@@ -568,8 +572,9 @@ static int hidx_sorted_setup(bt_dgraph* dg, SortedChunks* sc) {
 
 // The rounds as a captured graph of K rounds (K even), launched until a
 // round without a change; *rounds receives the rounds counted on the device.
-static int hidx_graph_loop(bt_dgraph* dg, uint64_t* mv, const SortedChunks& sc, unsigned long long* changed,
-                           uint8_t* const dirty[2], size_t nbm, unsigned grid_mv, unsigned grid_e, uint32_t* rounds) {
+static int hidx_graph_loop(bt_dgraph* dg, uint64_t* mv, const SortedChunks& sc, uint8_t* emark,
+                           unsigned long long* changed, uint8_t* const dirty[2], size_t nbm, unsigned grid_mv,
+                           unsigned grid_e, uint32_t* rounds) {
     bt_ctx* c = dg->ctx;
     hipStream_t s = c->stream;
     bt_index& ix = c->ix;
@@ -593,7 +598,8 @@ static int hidx_graph_loop(bt_dgraph* dg, uint64_t* mv, const SortedChunks& sc, 
         k_hidx_clear<<<grid_clear, 256, 0, s>>>(out, nbm, st.done);
         if (R)
             k_hidx_mv<<<grid_mv, 256, 0, s>>>(R, static_cast<const uint64_t*>(ix.vals1.ptr), prev, mv, sc.smv,
-                                              static_cast<const uint32_t*>(ix.bid.ptr), in, st.round, st.done);
+                                              static_cast<const uint32_t*>(ix.bid.ptr), in, emark, st.round,
+                                              st.done);
         if (sc.n)
             k_hidx_sort<<<sc.grid, kST, 0, s>>>(sc.list, sc.n, static_cast<const uint32_t*>(ix.bstart.ptr), sc.smv,
                                                  in, st.round, st.done);
@@ -613,6 +619,7 @@ static int hidx_graph_loop(bt_dgraph* dg, uint64_t* mv, const SortedChunks& sc, 
                    0,
                    in,
                    out,
+                   emark,
                    st.round,
                    st.done};
         k_hidx_round<<<grid_e, kHT, 0, s>>>(a);
@@ -681,10 +688,14 @@ int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds_out) {
         uint8_t* dirty[2] = {static_cast<uint8_t*>(ix.hdirty.ptr), static_cast<uint8_t*>(ix.hdirty.ptr) + nbm};
         SortedChunks sc;
         BT_TRY(hidx_sorted_setup(dg, &sc));
+        // per-edge marks in the index build's fill cursors (u32 per edge + 1,
+        // free after the build)
+        uint8_t* emark = static_cast<uint8_t*>(ix.inc_cur.ptr);
+        BT_HIP_TRY(hipMemsetAsync(emark, 0, m, s));
         bool const progress = getenv("BT_PEEL_PROGRESS") != nullptr;
         auto const t_first = std::chrono::steady_clock::now();
         if (c->hidx_graph_rounds > 0) {
-            BT_TRY(hidx_graph_loop(dg, mv, sc, changed, dirty, nbm, grid_mv, grid_e, &rounds));
+            BT_TRY(hidx_graph_loop(dg, mv, sc, emark, changed, dirty, nbm, grid_mv, grid_e, &rounds));
             prev = dg->res;  // the last round left both buffers equal
         }
         while (c->hidx_graph_rounds == 0) {
@@ -694,7 +705,8 @@ int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds_out) {
             uint8_t const* in = rounds > 1 ? dirty[(rounds + 1) & 1] : nullptr;
             if (R)
                 k_hidx_mv<<<grid_mv, 256, 0, s>>>(R, static_cast<const uint64_t*>(ix.vals1.ptr), prev, mv, sc.smv,
-                                                  static_cast<const uint32_t*>(ix.bid.ptr), in, nullptr, nullptr);
+                                                  static_cast<const uint32_t*>(ix.bid.ptr), in, emark, nullptr,
+                                                  nullptr);
             if (sc.n)
                 k_hidx_sort<<<sc.grid, kST, 0, s>>>(sc.list, sc.n, static_cast<const uint32_t*>(ix.bstart.ptr),
                                                      sc.smv, in, nullptr, nullptr);
@@ -714,6 +726,7 @@ int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds_out) {
                        rounds,
                        in,
                        dirty[rounds & 1],
+                       emark,
                        nullptr,
                        nullptr};
             k_hidx_round<<<grid_e, kHT, 0, s>>>(a);
