@@ -71,6 +71,7 @@ constexpr uint32_t kChunk = 2048;     // larger blooms: mv values sorted in chun
 constexpr int kST = 256;              // block size of the chunk sort
 constexpr uint32_t kCacheInc = 256;   // incidences whose bloom data a wave keeps in LDS (in its histogram)
 constexpr uint32_t kSearchCost = 12;  // cost of a binary search in one chunk, in values streamed
+constexpr uint64_t kHeavyInc = 1024;  // edges with more incidences: a workgroup each (sorted chunks only)
 
 // Device-side loop state (control words of the graph): round counter,
 // change flag of the current round, done flag.
@@ -78,6 +79,7 @@ struct LoopState {
     unsigned long long* round;
     unsigned long long* changed;
     unsigned long long* done;
+    unsigned long long* work;  // next heavy edge of the round
 };
 
 __global__ void k_hidx_loop_init(LoopState st) {
@@ -93,6 +95,7 @@ __global__ void k_hidx_round_begin(LoopState st) {
     if (threadIdx.x == 0 && blockIdx.x == 0 && !*st.done) {
         *st.round += 1;
         *st.changed = 0;
+        *st.work = 0;
     }
 }
 
@@ -147,6 +150,17 @@ __global__ void k_hidx_chunks(uint32_t NB, const uint32_t* __restrict__ bstart, 
         unsigned long long const at = atomicAdd(cnt, (unsigned long long)nch);
         if (list)
             for (uint32_t c = 0; c < nch; c++) list[at + c] = (uint64_t(b) << 32) | c;
+    }
+}
+
+// The heavy edges (above kHeavyInc incidences, support below 2^32): counted
+// into *cnt (list null), then listed.
+__global__ void k_hidx_heavy_list(uint32_t m, const uint64_t* __restrict__ inc_off, const uint64_t* __restrict__ sup,
+                                  unsigned long long* cnt, uint32_t* __restrict__ list) {
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
+        if (inc_off[e + 1] - inc_off[e] <= kHeavyInc || sup[e] > 0xFFFFFFFFull) continue;
+        unsigned long long const at = atomicAdd(cnt, 1ull);
+        if (list) list[at] = e;
     }
 }
 
@@ -215,6 +229,13 @@ struct HidxArgs {
     // per edge: one of its blooms changed in the previous round (set by
     // k_hidx_mv, cleared here when the edge is evaluated)
     uint8_t* emark;
+    // edges above kHeavyInc incidences with a support below 2^32 (sorted
+    // chunks only): taken from the work counter by the first heavy_blocks
+    // workgroups, a workgroup per edge; the others skip them
+    const uint64_t* sup;
+    const uint32_t* heavy;
+    uint32_t n_heavy, heavy_blocks;
+    unsigned long long* heavy_next;
     // graph rounds: the device loop state (null: round and dirty_in above
     // hold); the round number is read from it, and round 1 ignores dirty_in
     const unsigned long long* dev_round;
@@ -323,21 +344,24 @@ __device__ __forceinline__ uint32_t item_count(const uint32_t* __restrict__ smv,
 // the wave. Each bloom with prev[partner] >= t gives its mv values >= t less
 // the edge's own middle (mv = min(prev[e], partner) >= t). The wave takes
 // 64 incidences at a time (the bloom data of the first kCacheInc from its
-// LDS cache: pp, base and k in three rows of kCacheInc) and spreads their
-// items (a small bloom, or one chunk of a larger one) over its lanes, so
-// that the chunks of one large bloom are searched side by side. s_incl,
-// s_base and s_k are the wave's 64-entry LDS rows.
+// LDS cache, if any: pp, base and k in three rows of kCacheInc) and
+// spreads their items (a small bloom, or one chunk of a larger one) over
+// its lanes, so that the chunks of one large bloom are searched side by
+// side. s_incl, s_base and s_k are the wave's 64-entry LDS rows. A wave of
+// a workgroup that shares the edge takes the 64 incidences from q0 + first
+// on, every stride, and returns its own share.
 __device__ uint64_t count_at_least(const HidxArgs& a, uint32_t e, uint64_t q0, uint64_t q1, uint32_t t,
-                                   const uint32_t* cache, uint32_t* s_incl, uint32_t* s_base, uint32_t* s_k) {
+                                   const uint32_t* cache, uint32_t* s_incl, uint32_t* s_base, uint32_t* s_k,
+                                   uint64_t first = 0, uint64_t stride = kWave) {
     int const lane = threadIdx.x & (kWave - 1);
     uint64_t c = 0, own = 0;
-    for (uint64_t b0 = q0; b0 < q1; b0 += kWave) {
+    for (uint64_t b0 = q0 + first; b0 < q1; b0 += stride) {
         uint64_t const q = b0 + lane;
         IncInfo x{0u, 0u, 0u};
         if (q < q1) {
             uint64_t const i = q - q0;
-            x = i < kCacheInc ? IncInfo{cache[i], cache[kCacheInc + i], cache[2 * kCacheInc + i]}
-                              : inc_info(a, e, q);
+            x = cache && i < kCacheInc ? IncInfo{cache[i], cache[kCacheInc + i], cache[2 * kCacheInc + i]}
+                                       : inc_info(a, e, q);
         }
         uint32_t const items = x.pp >= t ? (x.k <= kSmallBloom ? 1u : (x.k + kChunk - 1) / kChunk) : 0u;
         own += __builtin_popcountll(__ballot(items != 0));
@@ -428,6 +452,73 @@ __device__ bool hidx_sorted(const HidxArgs& a, uint32_t e, uint64_t q0, uint64_t
     return true;
 }
 
+// The heavy edges of a round (one workgroup): each edge taken from the work
+// counter is evaluated by the whole workgroup, its waves sharing the
+// incidences of every count; the search is that of hidx_sorted (without
+// the cache). s_part holds kHW partial counts.
+__device__ void hidx_heavy(const HidxArgs& a, uint32_t* s_incl, uint32_t* s_base, uint32_t* s_k, uint64_t* s_part) {
+    __shared__ uint32_t s_item;
+    int const lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    bool changed = false;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = uint32_t(atomicAdd(a.heavy_next, 1ull));
+        __syncthreads();
+        uint32_t const i = s_item;
+        __syncthreads();
+        if (i >= a.n_heavy) break;
+        uint32_t const e = a.heavy[i];
+        uint64_t const pe = a.prev[e];
+        if (pe == 0) {  // cpu_bitruss.cpp:128
+            if (threadIdx.x == 0) a.curr[e] = 0;
+            continue;
+        }
+        if (a.dirty_in) {
+            bool const mk = a.emark[e] != 0;
+            __syncthreads();
+            if (!mk) {
+                if (threadIdx.x == 0) a.curr[e] = pe;
+                continue;
+            }
+            if (threadIdx.x == 0) a.emark[e] = 0;
+        }
+        uint64_t const q0 = a.inc_off[e], q1 = a.inc_off[e + 1];
+        auto count = [&](uint32_t t) {
+            uint64_t const c = count_at_least(a, e, q0, q1, t, nullptr, s_incl, s_base, s_k, uint64_t(wv) * kWave, kHT);
+            if (lane == 0) s_part[wv] = c;
+            __syncthreads();
+            uint64_t tot = 0;
+            for (int w = 0; w < kHW; w++) tot += s_part[w];
+            __syncthreads();
+            return tot;
+        };
+        uint64_t const c0 = count(uint32_t(pe));
+        uint64_t h = pe;
+        if (c0 < pe) {  // the two-sided search of hidx_sorted
+            uint32_t lo = uint32_t(c0), hi = uint32_t(pe) - 1;
+            while (lo < hi) {
+                uint32_t const mid = lo + (hi - lo + 1) / 2;
+                uint64_t const cm = count(mid);
+                if (cm >= mid) {
+                    lo = mid;
+                    hi = cm < hi ? uint32_t(cm) : hi;
+                } else {
+                    hi = mid - 1;
+                    lo = cm > lo ? uint32_t(cm) : lo;
+                }
+            }
+            h = lo;
+        }
+        if (threadIdx.x == 0) a.curr[e] = h;
+        if (h != pe) {
+            changed = true;
+            for (uint64_t q = q0 + threadIdx.x; q < q1; q += kHT) a.dirty_out[a.bid[a.inc[q]] - 1] = 1;
+            if (a.egraph && threadIdx.x == 0)
+                __hip_atomic_store(&a.glast[a.egraph[e]], a.round, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    if (changed && threadIdx.x == 0) __hip_atomic_store(a.changed, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __global__ __launch_bounds__(kHT) void k_hidx_round(HidxArgs a) {
     if (a.done) {
         if (*a.done) return;
@@ -439,16 +530,21 @@ __global__ __launch_bounds__(kHT) void k_hidx_round(HidxArgs a) {
     __shared__ uint64_t s_pp[kHW][kWave];
     int const lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     uint32_t* hist = s_hist[wv];
-    uint32_t const nw = gridDim.x * kHW;
     bool changed = false;
+    if (blockIdx.x < a.heavy_blocks) {
+        hidx_heavy(a, s_incl[wv], s_base[wv], s_skip[wv], reinterpret_cast<uint64_t*>(s_pp));
+        return;
+    }
+    uint32_t const nw = (gridDim.x - a.heavy_blocks) * kHW;
     uint32_t last_g = ~0u;  // graph whose change this wave last recorded
-    for (uint32_t e = blockIdx.x * kHW + wv; e < a.m; e += nw) {
+    for (uint32_t e = (blockIdx.x - a.heavy_blocks) * kHW + wv; e < a.m; e += nw) {
+        uint64_t const q0 = a.inc_off[e], q1 = a.inc_off[e + 1];
+        if (a.n_heavy && q1 - q0 > kHeavyInc && a.sup[e] <= 0xFFFFFFFFull) continue;  // a heavy edge
         uint64_t const pe = a.prev[e];
         if (pe == 0) {  // cpu_bitruss.cpp:128
             if (lane == 0) a.curr[e] = 0;
             continue;
         }
-        uint64_t const q0 = a.inc_off[e], q1 = a.inc_off[e + 1];
         // no bloom of e saw a change in the previous round: the values of
         // e's butterflies, and so its h-index, are those of that round
         if (a.dirty_in) {
@@ -535,6 +631,8 @@ struct SortedChunks {
     const uint64_t* list = nullptr;
     uint64_t n = 0;
     unsigned grid = 1;
+    const uint32_t* heavy = nullptr;  // the heavy edges and their workgroups
+    uint32_t n_heavy = 0, heavy_blocks = 0;
 };
 
 // smv lives in the peel's claim stamps (bclaim, at least R u32) and the
@@ -567,6 +665,26 @@ static int hidx_sorted_setup(bt_dgraph* dg, SortedChunks* sc) {
         sc->n = n;
         sc->grid = unsigned(std::min<uint64_t>(n, uint64_t(c->cu_count) * 16));
     }
+    // the heavy edges, listed in the peel's per-block live-edge segments
+    // (live, u32 per edge, unused by h-index rounds)
+    uint32_t const m = dg->m;
+    auto const* inc_off = static_cast<const uint64_t*>(ix.inc_off.ptr);
+    unsigned const grid_m = grid_for(std::max<uint32_t>(m, 1), 256, unsigned(c->cu_count) * 16);
+    BT_HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(uint64_t), s));
+    k_hidx_heavy_list<<<grid_m, 256, 0, s>>>(m, inc_off, dg->sup, cnt, nullptr);
+    BT_HIP_TRY(hipGetLastError());
+    BT_HIP_TRY(hipMemcpyAsync(h, cnt, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    BT_HIP_TRY(hipStreamSynchronize(s));
+    uint32_t const nh = uint32_t(h[0]);
+    if (nh) {
+        BT_TRY(bt_buf_reserve(ix.live, sizeof(uint32_t) * nh));
+        BT_HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(uint64_t), s));
+        k_hidx_heavy_list<<<grid_m, 256, 0, s>>>(m, inc_off, dg->sup, cnt, static_cast<uint32_t*>(ix.live.ptr));
+        BT_HIP_TRY(hipGetLastError());
+        sc->heavy = static_cast<const uint32_t*>(ix.live.ptr);
+        sc->n_heavy = nh;
+        sc->heavy_blocks = std::min<uint32_t>(nh, uint32_t(c->cu_count));
+    }
     return BT_OK;
 }
 
@@ -581,7 +699,8 @@ static int hidx_graph_loop(bt_dgraph* dg, uint64_t* mv, const SortedChunks& sc, 
     uint32_t const R = uint32_t(ix.R), m = dg->m;
     int const K = c->hidx_graph_rounds + (c->hidx_graph_rounds & 1);
     LoopState const st{reinterpret_cast<unsigned long long*>(dg->ctl + 17), changed,
-                       reinterpret_cast<unsigned long long*>(dg->ctl + 18)};
+                       reinterpret_cast<unsigned long long*>(dg->ctl + 18),
+                       reinterpret_cast<unsigned long long*>(dg->ctl + 21)};
     k_hidx_loop_init<<<1, 64, 0, s>>>(st);
     BT_HIP_TRY(hipGetLastError());
     unsigned const grid_clear = grid_for(nbm, 256, unsigned(c->cu_count) * 8);
@@ -620,9 +739,14 @@ static int hidx_graph_loop(bt_dgraph* dg, uint64_t* mv, const SortedChunks& sc, 
                    in,
                    out,
                    emark,
+                   dg->sup,
+                   sc.heavy,
+                   sc.n_heavy,
+                   sc.heavy_blocks,
+                   st.work,
                    st.round,
                    st.done};
-        k_hidx_round<<<grid_e, kHT, 0, s>>>(a);
+        k_hidx_round<<<grid_e + sc.heavy_blocks, kHT, 0, s>>>(a);
         k_hidx_round_end<<<1, 64, 0, s>>>(st);
     }
     hipError_t const ce = hipStreamEndCapture(s, &graph);
@@ -679,6 +803,7 @@ int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds_out) {
         uint64_t* prev = dg->res;
         uint64_t* curr = dg->aux;
         unsigned long long* changed = reinterpret_cast<unsigned long long*>(dg->ctl + 16);
+        unsigned long long* work = reinterpret_cast<unsigned long long*>(dg->ctl + 21);  // next heavy edge
         unsigned const grid_mv = grid_for(std::max<uint32_t>(R, 1), 256, unsigned(c->cu_count) * 16);
         unsigned const grid_e = std::max(1u, std::min(unsigned(c->cu_count) * 2, (m + kHW - 1) / kHW));
         uint64_t* h_changed = static_cast<uint64_t*>(c->h_pinned);
@@ -701,6 +826,7 @@ int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds_out) {
         while (c->hidx_graph_rounds == 0) {
             rounds++;
             BT_HIP_TRY(hipMemsetAsync(changed, 0, sizeof(uint64_t), s));
+            BT_HIP_TRY(hipMemsetAsync(work, 0, sizeof(uint64_t), s));
             BT_HIP_TRY(hipMemsetAsync(dirty[rounds & 1], 0, nbm, s));
             uint8_t const* in = rounds > 1 ? dirty[(rounds + 1) & 1] : nullptr;
             if (R)
@@ -727,9 +853,14 @@ int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds_out) {
                        in,
                        dirty[rounds & 1],
                        emark,
+                       dg->sup,
+                       sc.heavy,
+                       sc.n_heavy,
+                       sc.heavy_blocks,
+                       work,
                        nullptr,
                        nullptr};
-            k_hidx_round<<<grid_e, kHT, 0, s>>>(a);
+            k_hidx_round<<<grid_e + sc.heavy_blocks, kHT, 0, s>>>(a);
             BT_HIP_TRY(hipGetLastError());
             BT_HIP_TRY(hipMemcpyAsync(h_changed, changed, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
             BT_HIP_TRY(hipStreamSynchronize(s));

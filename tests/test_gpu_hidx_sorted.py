@@ -3,7 +3,8 @@
 those of the rounds that stream every value and the CPU oracle's
 (compute_bitruss_hindex, cpu_bitruss.cpp:200-239; the reference's GPU loop,
 bitruss_hidx.cu:51-256), on graphs whose blooms span one chunk of 2,048
-values and several, with the rounds one by one and as captured graphs, in
+values and several and with edges in thousands of blooms (evaluated by a
+whole workgroup), with the rounds one by one and as captured graphs, in
 batches, and on the 2M-edge golden."""
 from __future__ import annotations
 
@@ -45,8 +46,24 @@ def twin_hubs(seed: int, p_hub, nv: int, nu: int, p: float) -> gb.Graph:
     return G.from_pairs(pairs)
 
 
+def two_sided_hubs(seed: int, nu: int, nv: int, hu: int, pu: float, hv: int, pv: float, p: float) -> gb.Graph:
+    """hu upper hubs adjacent to each lower vertex with probability pu, hv
+    lower hubs adjacent to each upper vertex with probability pv, and random
+    edges of density p: the edges between two hubs lie in thousands of
+    blooms (the heavy edges, a workgroup each)."""
+    rng = np.random.default_rng(seed)
+    pairs = set()
+    for h in range(hu):
+        pairs |= {(h, int(j)) for j in np.nonzero(rng.random(nv) < pu)[0]}
+    for h in range(hv):
+        pairs |= {(int(i), h) for i in np.nonzero(rng.random(nu) < pv)[0]}
+    pairs |= {(int(i), int(j)) for i, j in np.argwhere(rng.random((nu, nv)) < p)}
+    return G.from_pairs(sorted(pairs))
+
+
 def _graphs():
-    gs = [("twin_hubs1", twin_hubs(1, [0.9, 0.9, 0.85, 0.3], 3000, 200, 0.01)),
+    gs = [("two_sided_hubs", two_sided_hubs(7, 3000, 3000, 3, 0.85, 3, 0.8, 0.002)),
+          ("twin_hubs1", twin_hubs(1, [0.9, 0.9, 0.85, 0.3], 3000, 200, 0.01)),
           ("twin_hubs2", twin_hubs(2, [0.95, 0.7, 0.5, 0.5, 0.1], 2600, 300, 0.02)),
           ("hub", G.hub_graph(5, 6000, 20000, 400, 7000)),
           ("K3x2100", G.complete(3, 2100)),
@@ -80,6 +97,7 @@ def test_sorted_in_batches(dev):
     graph's results and round count equal its plain run that streams."""
     graphs = [gb.Graph.generate("mix_graph", i) for i in range(6)]
     graphs.insert(2, twin_hubs(3, [0.9, 0.9, 0.4], 2500, 150, 0.02))
+    graphs.insert(4, two_sided_hubs(8, 2500, 2500, 2, 0.9, 2, 0.8, 0.002))
     with gb.Device(dev.device) as d:
         d.set_hidx_sorted(1)
         b = d.upload_batch(graphs)
