@@ -25,12 +25,14 @@
 //
 // Sorted chunks: that pass costs the sizes of the edge's blooms, which on a
 // skewed graph run to tens of thousands of middles per bloom. Each round
-// therefore also keeps, for every bloom above kSmallBloom middles, its mv
-// values (clamped to 32 bits, exact for thresholds below 2^32) sorted in
-// descending chunks of kChunk, re-sorting only the blooms with a change in
-// the previous round. The number of an edge's values >= t is then, per
-// bloom with prev[partner] >= t, a binary search per chunk (a pass for the
-// small blooms) minus the edge's own middle, and the h-index is found by a
+// therefore also keeps every bloom's mv values (clamped to 32 bits, exact
+// for thresholds below 2^32) in descending order: whole for blooms up to
+// kSmallBloom middles (a thread sorts one), in chunks of kChunk above (a
+// workgroup sorts one), re-sorting only the blooms with a change in the
+// previous round. The number of an edge's values >= t is then, per bloom
+// with prev[partner] >= t, a binary search per chunk (the chunks of one
+// bloom spread over the wave) minus the edge's own middle, and the h-index
+// is found by a
 // search over t after a first count at t = prev[e] (most edges keep their
 // value in the late rounds), each count narrowing the range from both
 // sides. An edge none of whose blooms changed in the previous round keeps
@@ -66,7 +68,7 @@ namespace {
 constexpr int kHT = 512;              // block size
 constexpr int kHW = kHT / kWave;      // waves per block
 constexpr uint32_t kHB = 1024;        // histogram bins per wave
-constexpr uint32_t kSmallBloom = 32;  // blooms up to this size: counted by a pass over their values
+constexpr uint32_t kSmallBloom = 32;  // blooms up to this size: sorted whole, a thread each
 constexpr uint32_t kChunk = 2048;     // larger blooms: mv values sorted in chunks of this length
 constexpr int kST = 256;              // block size of the chunk sort
 constexpr uint32_t kCacheInc = 256;   // incidences whose bloom data a wave keeps in LDS (in its histogram)
@@ -161,6 +163,34 @@ __global__ void k_hidx_heavy_list(uint32_t m, const uint64_t* __restrict__ inc_o
         if (inc_off[e + 1] - inc_off[e] <= kHeavyInc || sup[e] > 0xFFFFFFFFull) continue;
         unsigned long long const at = atomicAdd(cnt, 1ull);
         if (list) list[at] = e;
+    }
+}
+
+// Sorts the blooms of 2..kSmallBloom middles of smv in descending order, a
+// thread per bloom (insertion sort in its LDS row), skipping the blooms
+// without a change in the previous round.
+__global__ __launch_bounds__(256) void k_hidx_sort_small(uint32_t NB, const uint32_t* __restrict__ bstart,
+                                                         uint32_t* __restrict__ smv,
+                                                         const uint8_t* __restrict__ dirty_in,
+                                                         const unsigned long long* dev_round,
+                                                         const unsigned long long* done) {
+    if (done && *done) return;
+    if (dev_round && *dev_round == 1) dirty_in = nullptr;
+    __shared__ uint32_t s_rows[256 * (kSmallBloom + 1)];
+    uint32_t* v = s_rows + threadIdx.x * (kSmallBloom + 1);
+    for (size_t b = blockIdx.x * size_t(blockDim.x) + threadIdx.x; b < NB; b += size_t(gridDim.x) * blockDim.x) {
+        uint32_t const b0 = bstart[b], k = bstart[b + 1] - b0;
+        if (k < 2 || k > kSmallBloom || (dirty_in && !dirty_in[b])) continue;
+        for (uint32_t j = 0; j < k; j++) {
+            uint32_t const x = smv[b0 + j];
+            uint32_t i = j;
+            while (i > 0 && v[i - 1] < x) {
+                v[i] = v[i - 1];
+                i--;
+            }
+            v[i] = x;
+        }
+        for (uint32_t j = 0; j < k; j++) smv[b0 + j] = v[j];
     }
 }
 
@@ -319,15 +349,10 @@ __device__ __forceinline__ IncInfo inc_info(const HidxArgs& a, uint32_t e, uint6
     return x;
 }
 
-// mv values >= t in one item of a bloom: the whole bloom when it is small
-// (a pass), else its chunk ch (a binary search in descending order).
+// mv values >= t in one item of a bloom: the whole bloom when it is small,
+// else its chunk ch (a binary search in descending order either way).
 __device__ __forceinline__ uint32_t item_count(const uint32_t* __restrict__ smv, uint32_t base, uint32_t k,
                                                uint32_t ch, uint32_t t) {
-    if (k <= kSmallBloom) {
-        uint32_t c = 0;
-        for (uint32_t j = 0; j < k; j++) c += smv[base + j] >= t ? 1u : 0u;
-        return c;
-    }
     const uint32_t* s = smv + base + ch * kChunk;
     uint32_t lo = 0, hi = std::min(kChunk, k - ch * kChunk);
     while (lo < hi) {  // the first position below t
@@ -415,7 +440,9 @@ __device__ bool hidx_sorted(const HidxArgs& a, uint32_t e, uint64_t q0, uint64_t
             }
             if (x.pp) {
                 vals += x.k - 1;
-                cost += x.k <= kSmallBloom ? x.k : uint64_t((x.k + kChunk - 1) / kChunk) * kSearchCost;
+                // a binary search per item: about log2 of its length, plus one
+                cost += x.k <= kSmallBloom ? uint64_t(33 - __builtin_clz(x.k))
+                                           : uint64_t((x.k + kChunk - 1) / kChunk) * kSearchCost;
             }
         }
     }
@@ -631,6 +658,7 @@ struct SortedChunks {
     const uint64_t* list = nullptr;
     uint64_t n = 0;
     unsigned grid = 1;
+    unsigned grid_small = 1;          // the small blooms' sort
     const uint32_t* heavy = nullptr;  // the heavy edges and their workgroups
     uint32_t n_heavy = 0, heavy_blocks = 0;
 };
@@ -656,6 +684,7 @@ static int hidx_sorted_setup(bt_dgraph* dg, SortedChunks* sc) {
     BT_HIP_TRY(hipStreamSynchronize(s));
     uint64_t const n = h[0];
     sc->smv = static_cast<uint32_t*>(ix.bclaim.ptr);
+    sc->grid_small = grid_for(NB, 256, unsigned(c->cu_count) * 8);
     if (n) {
         BT_TRY(bt_buf_reserve(ix.touched, sizeof(uint64_t) * n));
         BT_HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(uint64_t), s));
@@ -722,6 +751,9 @@ static int hidx_graph_loop(bt_dgraph* dg, uint64_t* mv, const SortedChunks& sc, 
         if (sc.n)
             k_hidx_sort<<<sc.grid, kST, 0, s>>>(sc.list, sc.n, static_cast<const uint32_t*>(ix.bstart.ptr), sc.smv,
                                                  in, st.round, st.done);
+        if (sc.smv)
+            k_hidx_sort_small<<<sc.grid_small, 256, 0, s>>>(uint32_t(ix.NB), static_cast<const uint32_t*>(ix.bstart.ptr),
+                                                            sc.smv, in, st.round, st.done);
         HidxArgs a{m,
                    prev,
                    curr,
@@ -836,6 +868,9 @@ int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds_out) {
             if (sc.n)
                 k_hidx_sort<<<sc.grid, kST, 0, s>>>(sc.list, sc.n, static_cast<const uint32_t*>(ix.bstart.ptr),
                                                      sc.smv, in, nullptr, nullptr);
+            if (sc.smv)
+                k_hidx_sort_small<<<sc.grid_small, 256, 0, s>>>(
+                    uint32_t(ix.NB), static_cast<const uint32_t*>(ix.bstart.ptr), sc.smv, in, nullptr, nullptr);
             HidxArgs a{m,
                        prev,
                        curr,
