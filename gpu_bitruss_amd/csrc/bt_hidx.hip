@@ -73,7 +73,7 @@ constexpr uint32_t kChunk = 2048;     // larger blooms: mv values sorted in chun
 constexpr int kST = 256;              // block size of the chunk sort
 constexpr uint32_t kCacheInc = 256;   // incidences whose bloom data a wave keeps in LDS (in its histogram)
 constexpr uint32_t kSearchCost = 12;  // cost of a binary search in one chunk, in values streamed
-constexpr uint64_t kHeavyInc = 1024;  // edges with more incidences: a workgroup each (sorted chunks only)
+constexpr uint64_t kHeavyInc = 256;   // edges with more incidences: a workgroup each (sorted chunks only)
 
 // Device-side loop state (control words of the graph): round counter,
 // change flag of the current round, done flag.
