@@ -33,7 +33,16 @@ against or_msp on the graphs the latter finishes and on the prefix above)
 instead of the per-pivot oracle, whose merges take hours on the 60M-edge
 graph:
 
-    python tests/golden/make_golden.py skewed_60m --skip-hidx --msp-bloom
+    python tests/golden/make_golden.py skewed_60m --skip-hidx --msp-bloom \
+        --msp-checkpoints 800,1600,3200,6400,12800,25600,51200,102400,150000,204800,250000
+
+(3 h 18 min on 8 threads; --msp-checkpoints also records the state after
+each listed level on the way). --peel-bloom records the bitruss numbers of
+the exact bloom peel (BLOOM_EXACT, pinned against the textbook peel), the
+fixed point of the h-index rounds, for graphs whose hidx oracle would take
+too long:
+
+    python tests/golden/make_golden.py skewed_60m --skip-hidx --skip-msp --peel-bloom
 
 Fields of an existing file that a run does not recompute are kept.
 """
@@ -68,6 +77,10 @@ def main() -> None:
     ap.add_argument("--skip-msp", action="store_true")
     ap.add_argument("--msp-bloom", action="store_true",
                     help="msp by the bloom restatement (or_msp_bloom) instead of or_msp")
+    ap.add_argument("--msp-checkpoints", default="", metavar="L1,L2,...",
+                    help="with --msp-bloom: also record the state after each listed level")
+    ap.add_argument("--peel-bloom", action="store_true",
+                    help="record the exact bloom peel's bitruss numbers (peel_sha256)")
     ap.add_argument("--msp-prefix", type=int, default=0, metavar="L",
                     help="also record the msp state after levels 1..L")
     ap.add_argument("--msp-from", nargs=2, metavar=("JSON", "U64"),
@@ -92,10 +105,27 @@ def main() -> None:
             "w_cnt": st["w_cnt"],
         })
         if not args.skip_msp and args.msp_bloom:
-            res, levels, complete, st2 = oracle.msp_bloom(g, args.threads)
+            cks = [int(x) for x in args.msp_checkpoints.split(",") if x]
+            ck = dict(out.get("msp_prefix_checkpoints", {}))
+
+            def on_state(level, state):
+                d = digest(state)
+                assert ck.get(str(level), d) == d, level
+                ck[str(level)] = d
+
+            res, levels, complete, st2 = oracle.msp_bloom_checkpoints(g, args.threads, cks, on_state, 0)
             assert complete
-            out.update({"msp_levels": levels, "msp_sha256": digest(res), "msp_stats": st2,
-                        "msp_oracle": "or_msp_bloom"})
+            out.update({"msp_levels": levels, "msp_sha256": digest(res), "msp_oracle": "or_msp_bloom",
+                        "msp_stats": {k: st2[k] for k in ("w_peel", "i_peel", "b_peel", "d_peel", "sum_curr",
+                                                          "sub_rounds")}})
+            if ck:
+                out["msp_prefix_checkpoints"] = ck
+        if args.peel_bloom:
+            res, levels, complete, st5 = oracle.msp_bloom(g, args.threads, 0, oracle.BLOOM_EXACT)
+            assert complete
+            out.update({"peel_sha256": digest(res), "peel_kmax": int(res.max()) if g.m else 0,
+                        "peel_levels": levels, "peel_oracle": "or_msp_bloom (BLOOM_EXACT)",
+                        "peel_stats": {k: st5[k] for k in ("w_peel", "b_peel", "sum_curr", "sub_rounds")}})
         elif not args.skip_msp:
             res, levels, st2 = oracle.msp(g, args.threads)
             out.update({"msp_levels": levels, "msp_sha256": digest(res), "msp_stats": st2, "msp_oracle": "or_msp"})

@@ -407,37 +407,63 @@ def test_skewed_60m_msp_prefix(dev):
         dg.free()
 
 
-@pytest.mark.gpu
-@pytest.mark.timeout(1000)
-def test_skewed_60m_msp_whole(dev, tmp_path):
-    """The 60M-edge graph's whole msp decomposition through the drop-in
-    command line (`coh --gpu --algo msp --dump`, main.cu:96-129) against the
-    bloom oracle's digest and level count (or_msp_bloom,
-    cpu_bitruss.cpp:536-597). The peel reports each time slice (about every
-    2 s) to a log under gpurun_out/, so the run shows its progress while the
+def _coh_skewed_60m(algo: str, tmp_path):
+    """Runs `coh --gpu --algo <algo> --dump --json` (main.cu:96-129) on the
+    60M-edge graph and returns its results and JSON summary. The decomposition
+    reports its progress (each msp time slice, each graph of h-index rounds)
+    to a log under gpurun_out/, so a long run shows it is alive while the
     test's own output is held by pytest."""
-    gold = json.load(open(os.path.join(GOLDEN, "skewed_60m.json")))
-    if "msp_sha256" not in gold:
-        pytest.skip("the oracle's whole msp of skewed_60m is not recorded (make_golden.py --msp-bloom)")
     import subprocess
 
     from gpu_bitruss_amd import _lib
 
     out_dir = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
     os.makedirs(out_dir, exist_ok=True)
-    log_path = os.path.join(out_dir, "skewed_60m_msp_progress.log")
-    binf, dump = str(tmp_path / "skewed_60m.bin"), str(tmp_path / "skewed_60m_msp.u64")
+    log_path = os.path.join(out_dir, f"skewed_60m_{algo}_progress.log")
+    binf, dump = str(tmp_path / "skewed_60m.bin"), str(tmp_path / f"skewed_60m_{algo}.u64")
     with open(log_path, "a") as log:
         log.write("[skewed_60m] generating the graph\n")
         log.flush()
         gb.Graph.generate("skewed_60m").graph_to_bin(binf)
-        log.write("[skewed_60m] coh --gpu --algo msp\n")
+        log.write(f"[skewed_60m] coh --gpu --algo {algo}\n")
         log.flush()
         env = dict(os.environ, BT_PEEL_PROGRESS="1")
-        r = subprocess.run([_lib.COH_PATH, "--gpu", "--algo", "msp", "--graph", binf, "--dump", dump, "--json"],
+        r = subprocess.run([_lib.COH_PATH, "--gpu", "--algo", algo, "--graph", binf, "--dump", dump, "--json"],
                            stdout=subprocess.PIPE, stderr=log, text=True, env=env, timeout=900)
     assert r.returncode == 0, r.stdout
-    res = np.fromfile(dump, dtype="<u8")
     summary = json.loads([l for l in r.stdout.splitlines() if l.startswith("{")][0])
+    return np.fromfile(dump, dtype="<u8"), summary
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(1000)
+def test_skewed_60m_msp_whole(dev, tmp_path):
+    """The 60M-edge graph's whole msp decomposition through the drop-in
+    command line against the bloom oracle's digest and level count
+    (or_msp_bloom, cpu_bitruss.cpp:536-597)."""
+    gold = json.load(open(os.path.join(GOLDEN, "skewed_60m.json")))
+    if "msp_sha256" not in gold:
+        pytest.skip("the oracle's whole msp of skewed_60m is not recorded (make_golden.py --msp-bloom)")
+    res, summary = _coh_skewed_60m("msp", tmp_path)
     assert res.size == gold["m"]
     assert digest(res) == gold["msp_sha256"] and summary["iters"] == gold["msp_levels"]
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(1000)
+def test_skewed_60m_hidx_whole(dev, tmp_path):
+    """The 60M-edge graph's h-index decomposition to its fixed point
+    (`coh --gpu --algo hidx`, bitruss_hidx.cu:179-256; about 300 s with the
+    sorted chunks) against the bitruss numbers of the oracle's exact bloom
+    peel (or_msp_bloom with BLOOM_EXACT, which equals the textbook peel
+    cpu_bitruss.cpp:34-122 on every graph the CPU tests hold): the Jacobi
+    rounds' fixed point is the bitruss decomposition, as the 2M-edge golden
+    shows for the oracle's own hidx. The round count (135 on MI355X, the same
+    with and without sorted chunks on every graph tested) has no oracle
+    here: the oracle's Jacobi rounds would stream 4.4e12 values each."""
+    gold = json.load(open(os.path.join(GOLDEN, "skewed_60m.json")))
+    if "peel_sha256" not in gold:
+        pytest.skip("the oracle's exact peel of skewed_60m is not recorded")
+    res, summary = _coh_skewed_60m("hidx", tmp_path)
+    assert res.size == gold["m"]
+    assert digest(res) == gold["peel_sha256"] and int(res.max()) == gold["peel_kmax"]

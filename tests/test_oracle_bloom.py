@@ -108,3 +108,20 @@ def test_exact_mode_on_powerlaw_2m_golden():
     res, levels, complete, _ = oracle.msp_bloom(g, os.cpu_count() or 4, 0, oracle.BLOOM_EXACT)
     assert complete and levels == gold["hidx_kmax"]
     assert hashlib.sha256(np.ascontiguousarray(res, dtype="<u8").tobytes()).hexdigest() == gold["hidx_sha256"]
+
+
+def test_recorded_exact_peels_equal_recorded_hidx():
+    """Where a golden file holds both the exact bloom peel (peel_sha256) and
+    the per-pivot oracle's h-index rounds (hidx_sha256), they agree: the
+    fixed point of the Jacobi rounds is the bitruss decomposition. The
+    60M-edge graph's h-index test relies on this (its Jacobi rounds have no
+    oracle run)."""
+    both = 0
+    for name in sorted(os.listdir(GOLDEN)):
+        if not name.endswith(".json"):
+            continue
+        gold = json.load(open(os.path.join(GOLDEN, name)))
+        if "peel_sha256" in gold and "hidx_sha256" in gold:
+            both += 1
+            assert gold["peel_sha256"] == gold["hidx_sha256"] and gold["peel_kmax"] == gold["hidx_kmax"], name
+    assert both >= 1
