@@ -106,37 +106,67 @@ __global__ void k_hidx_round_end(LoopState st) {
     if (threadIdx.x == 0 && blockIdx.x == 0 && !*st.done && !*st.changed) *st.done = 1;
 }
 
-// Clears a round's bloom marks (a kernel, not a memset node: the graph
-// replays it).
-__global__ void k_hidx_clear(uint8_t* __restrict__ p, size_t n, const unsigned long long* done) {
+// Clears a round's bloom marks and largest changed values (a kernel, not a
+// memset node: the graph replays it).
+__global__ void k_hidx_clear(uint8_t* __restrict__ p, uint32_t* __restrict__ q, size_t n,
+                             const unsigned long long* done) {
     if (done && *done) return;
-    for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
         p[i] = 0;
+        q[i] = 0;
+    }
 }
 
 // mv of every record, and its 32-bit copy smv (null: none) for the sorted
 // chunks. A bloom without a change in the previous round (dirty_in, null:
-// every bloom) keeps the values it has; the two edges of each record of a
-// bloom with a change get their mark in emark, which tells the round which
-// edges to evaluate again (an edge lies in a bloom exactly when it is an
-// edge of one of its records). In graph rounds round 1 ignores dirty_in.
+// every bloom) keeps the values it has; for a bloom with a change, dmax
+// receives the largest old mv among its records whose mv changed (clamped
+// to 32 bits, 0xFFFFFFFF standing for any larger value). In graph rounds
+// round 1 ignores dirty_in.
 __global__ void k_hidx_mv(uint32_t R, const uint64_t* __restrict__ rec, const uint64_t* __restrict__ prev,
                           uint64_t* __restrict__ mv, uint32_t* __restrict__ smv, const uint32_t* __restrict__ bid,
-                          const uint8_t* __restrict__ dirty_in, uint8_t* __restrict__ emark,
+                          const uint8_t* __restrict__ dirty_in, uint32_t* __restrict__ dmax,
                           const unsigned long long* dev_round, const unsigned long long* done) {
     if (done && *done) return;
     if (dev_round && *dev_round == 1) dirty_in = nullptr;
     for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < R; p += uint64_t(gridDim.x) * blockDim.x) {
-        if (dirty_in && !dirty_in[bid[p] - 1]) continue;
+        uint32_t const b0 = dirty_in ? bid[p] - 1 : 0u;
+        if (dirty_in && !dirty_in[b0]) continue;
         uint64_t const r = rec[p];
-        if (dirty_in) {
-            emark[uint32_t(r >> 32)] = 1;
-            emark[uint32_t(r)] = 1;
-        }
         uint64_t const a = prev[uint32_t(r >> 32)], b = prev[uint32_t(r)];
         uint64_t const v = a < b ? a : b;
+        if (dirty_in) {
+            uint64_t const old = mv[p];
+            if (old != v) atomicMax(&dmax[b0], old < 0xFFFFFFFFull ? uint32_t(old) : 0xFFFFFFFFu);
+        }
         mv[p] = v;
         if (smv) smv[p] = v < 0xFFFFFFFFull ? uint32_t(v) : 0xFFFFFFFFu;
+    }
+}
+
+// Marks the edges to evaluate again: both edges x of every record of a
+// bloom whose largest changed old value d is >= prev[x]. An edge's count at
+// t = prev[x] changes only when one of its butterflies' values drops from
+// >= t to < t: either a middle's mv did (its old mv >= t) or the partner's
+// value did, which changes the edge's own middle from an old mv >= t. Its
+// count at prev[x] stays >= prev[x] otherwise, and so does its h-index.
+// (An edge lies in a bloom exactly when it is an edge of one of its
+// records.) Round 1 evaluates every edge.
+__global__ void k_hidx_mark(uint32_t R, const uint64_t* __restrict__ rec, const uint64_t* __restrict__ prev,
+                            const uint32_t* __restrict__ bid, const uint8_t* __restrict__ dirty_in,
+                            const uint32_t* __restrict__ dmax, uint8_t* __restrict__ emark,
+                            const unsigned long long* dev_round, const unsigned long long* done) {
+    if (done && *done) return;
+    if (!dirty_in || (dev_round && *dev_round == 1)) return;
+    for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < R; p += uint64_t(gridDim.x) * blockDim.x) {
+        uint32_t const b0 = bid[p] - 1;
+        if (!dirty_in[b0]) continue;
+        uint32_t const d = dmax[b0];
+        if (d == 0) continue;
+        uint64_t const r = rec[p];
+        uint32_t const x = uint32_t(r >> 32), y = uint32_t(r);
+        if (d == 0xFFFFFFFFu || d >= prev[x]) emark[x] = 1;
+        if (d == 0xFFFFFFFFu || d >= prev[y]) emark[y] = 1;
     }
 }
 
@@ -720,8 +750,8 @@ static int hidx_sorted_setup(bt_dgraph* dg, SortedChunks* sc) {
 // The rounds as a captured graph of K rounds (K even), launched until a
 // round without a change; *rounds receives the rounds counted on the device.
 static int hidx_graph_loop(bt_dgraph* dg, uint64_t* mv, const SortedChunks& sc, uint8_t* emark,
-                           unsigned long long* changed, uint8_t* const dirty[2], size_t nbm, unsigned grid_mv,
-                           unsigned grid_e, uint32_t* rounds) {
+                           uint32_t* const dmax[2], unsigned long long* changed, uint8_t* const dirty[2], size_t nbm,
+                           unsigned grid_mv, unsigned grid_e, uint32_t* rounds) {
     bt_ctx* c = dg->ctx;
     hipStream_t s = c->stream;
     bt_index& ix = c->ix;
@@ -743,11 +773,15 @@ static int hidx_graph_loop(bt_dgraph* dg, uint64_t* mv, const SortedChunks& sc, 
         uint8_t* out = dirty[(k + 1) & 1];  // round r writes dirty[r & 1]
         uint8_t* in = dirty[k & 1];         // and reads dirty[(r + 1) & 1]
         k_hidx_round_begin<<<1, 64, 0, s>>>(st);
-        k_hidx_clear<<<grid_clear, 256, 0, s>>>(out, nbm, st.done);
+        uint32_t* dm = dmax[(k + 1) & 1];  // round r's largest changed values: dmax[r & 1]
+        k_hidx_clear<<<grid_clear, 256, 0, s>>>(out, dm, nbm, st.done);
         if (R)
             k_hidx_mv<<<grid_mv, 256, 0, s>>>(R, static_cast<const uint64_t*>(ix.vals1.ptr), prev, mv, sc.smv,
-                                              static_cast<const uint32_t*>(ix.bid.ptr), in, emark, st.round,
-                                              st.done);
+                                              static_cast<const uint32_t*>(ix.bid.ptr), in, dm, st.round, st.done);
+        if (R)
+            k_hidx_mark<<<grid_mv, 256, 0, s>>>(R, static_cast<const uint64_t*>(ix.vals1.ptr), prev,
+                                                static_cast<const uint32_t*>(ix.bid.ptr), in, dm, emark, st.round,
+                                                st.done);
         if (sc.n)
             k_hidx_sort<<<sc.grid, kST, 0, s>>>(sc.list, sc.n, static_cast<const uint32_t*>(ix.bstart.ptr), sc.smv,
                                                  in, st.round, st.done);
@@ -849,10 +883,15 @@ int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds_out) {
         // free after the build)
         uint8_t* emark = static_cast<uint8_t*>(ix.inc_cur.ptr);
         BT_HIP_TRY(hipMemsetAsync(emark, 0, m, s));
+        // per bloom and round parity: the largest old mv among the records
+        // whose mv changed, in the peel's live-middle counts (bk, u32 per
+        // record; a bloom has at least two records)
+        BT_TRY(bt_buf_reserve(ix.bk, sizeof(uint32_t) * 2 * nbm));
+        uint32_t* dmax[2] = {static_cast<uint32_t*>(ix.bk.ptr), static_cast<uint32_t*>(ix.bk.ptr) + nbm};
         bool const progress = getenv("BT_PEEL_PROGRESS") != nullptr;
         auto const t_first = std::chrono::steady_clock::now();
         if (c->hidx_graph_rounds > 0) {
-            BT_TRY(hidx_graph_loop(dg, mv, sc, emark, changed, dirty, nbm, grid_mv, grid_e, &rounds));
+            BT_TRY(hidx_graph_loop(dg, mv, sc, emark, dmax, changed, dirty, nbm, grid_mv, grid_e, &rounds));
             prev = dg->res;  // the last round left both buffers equal
         }
         while (c->hidx_graph_rounds == 0) {
@@ -860,11 +899,16 @@ int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds_out) {
             BT_HIP_TRY(hipMemsetAsync(changed, 0, sizeof(uint64_t), s));
             BT_HIP_TRY(hipMemsetAsync(work, 0, sizeof(uint64_t), s));
             BT_HIP_TRY(hipMemsetAsync(dirty[rounds & 1], 0, nbm, s));
+            BT_HIP_TRY(hipMemsetAsync(dmax[rounds & 1], 0, sizeof(uint32_t) * nbm, s));
             uint8_t const* in = rounds > 1 ? dirty[(rounds + 1) & 1] : nullptr;
             if (R)
                 k_hidx_mv<<<grid_mv, 256, 0, s>>>(R, static_cast<const uint64_t*>(ix.vals1.ptr), prev, mv, sc.smv,
-                                                  static_cast<const uint32_t*>(ix.bid.ptr), in, emark, nullptr,
-                                                  nullptr);
+                                                  static_cast<const uint32_t*>(ix.bid.ptr), in, dmax[rounds & 1],
+                                                  nullptr, nullptr);
+            if (R)
+                k_hidx_mark<<<grid_mv, 256, 0, s>>>(R, static_cast<const uint64_t*>(ix.vals1.ptr), prev,
+                                                    static_cast<const uint32_t*>(ix.bid.ptr), in, dmax[rounds & 1],
+                                                    emark, nullptr, nullptr);
             if (sc.n)
                 k_hidx_sort<<<sc.grid, kST, 0, s>>>(sc.list, sc.n, static_cast<const uint32_t*>(ix.bstart.ptr),
                                                      sc.smv, in, nullptr, nullptr);
