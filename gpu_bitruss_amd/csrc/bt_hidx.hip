@@ -74,6 +74,7 @@ constexpr int kST = 256;              // block size of the chunk sort
 constexpr uint32_t kCacheInc = 256;   // incidences whose bloom data a wave keeps in LDS (in its histogram)
 constexpr uint32_t kSearchCost = 12;  // cost of a binary search in one chunk, in values streamed
 constexpr uint64_t kHeavyInc = 256;   // edges with more incidences: a workgroup each (sorted chunks only)
+constexpr uint32_t kHeavyCache = kHW * kHB / 3;  // incidences a heavy edge's workgroup caches in its histograms
 
 // Device-side loop state (control words of the graph): round counter,
 // change flag of the current round, done flag.
@@ -399,7 +400,7 @@ __device__ __forceinline__ uint32_t item_count(const uint32_t* __restrict__ smv,
 // the wave. Each bloom with prev[partner] >= t gives its mv values >= t less
 // the edge's own middle (mv = min(prev[e], partner) >= t). The wave takes
 // 64 incidences at a time (the bloom data of the first kCacheInc from its
-// LDS cache, if any: pp, base and k in three rows of kCacheInc) and
+// LDS cache, if any: pp, base and k in three rows of cap) and
 // spreads their items (a small bloom, or one chunk of a larger one) over
 // its lanes, so that the chunks of one large bloom are searched side by
 // side. s_incl, s_base and s_k are the wave's 64-entry LDS rows. A wave of
@@ -407,7 +408,7 @@ __device__ __forceinline__ uint32_t item_count(const uint32_t* __restrict__ smv,
 // on, every stride, and returns its own share.
 __device__ uint64_t count_at_least(const HidxArgs& a, uint32_t e, uint64_t q0, uint64_t q1, uint32_t t,
                                    const uint32_t* cache, uint32_t* s_incl, uint32_t* s_base, uint32_t* s_k,
-                                   uint64_t first = 0, uint64_t stride = kWave) {
+                                   uint64_t first = 0, uint64_t stride = kWave, uint32_t cap = kCacheInc) {
     int const lane = threadIdx.x & (kWave - 1);
     uint64_t c = 0, own = 0;
     for (uint64_t b0 = q0 + first; b0 < q1; b0 += stride) {
@@ -415,8 +416,7 @@ __device__ uint64_t count_at_least(const HidxArgs& a, uint32_t e, uint64_t q0, u
         IncInfo x{0u, 0u, 0u};
         if (q < q1) {
             uint64_t const i = q - q0;
-            x = cache && i < kCacheInc ? IncInfo{cache[i], cache[kCacheInc + i], cache[2 * kCacheInc + i]}
-                                       : inc_info(a, e, q);
+            x = cache && i < cap ? IncInfo{cache[i], cache[cap + i], cache[2 * cap + i]} : inc_info(a, e, q);
         }
         uint32_t const items = x.pp >= t ? (x.k <= kSmallBloom ? 1u : (x.k + kChunk - 1) / kChunk) : 0u;
         own += __builtin_popcountll(__ballot(items != 0));
@@ -511,9 +511,11 @@ __device__ bool hidx_sorted(const HidxArgs& a, uint32_t e, uint64_t q0, uint64_t
 
 // The heavy edges of a round (one workgroup): each edge taken from the work
 // counter is evaluated by the whole workgroup, its waves sharing the
-// incidences of every count; the search is that of hidx_sorted (without
-// the cache). s_part holds kHW partial counts.
-__device__ void hidx_heavy(const HidxArgs& a, uint32_t* s_incl, uint32_t* s_base, uint32_t* s_k, uint64_t* s_part) {
+// incidences of every count, with the bloom data of its first kHeavyCache
+// incidences cached in the workgroup's histograms (cache); the search is
+// that of hidx_sorted. s_part holds kHW partial counts.
+__device__ void hidx_heavy(const HidxArgs& a, uint32_t* cache, uint32_t* s_incl, uint32_t* s_base, uint32_t* s_k,
+                           uint64_t* s_part) {
     __shared__ uint32_t s_item;
     int const lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     bool changed = false;
@@ -539,8 +541,18 @@ __device__ void hidx_heavy(const HidxArgs& a, uint32_t* s_incl, uint32_t* s_base
             if (threadIdx.x == 0) a.emark[e] = 0;
         }
         uint64_t const q0 = a.inc_off[e], q1 = a.inc_off[e + 1];
+        uint64_t const qc = std::min(q1, q0 + kHeavyCache);
+        for (uint64_t q = q0 + threadIdx.x; q < qc; q += kHT) {
+            IncInfo const x = inc_info(a, e, q);
+            uint64_t const i = q - q0;
+            cache[i] = x.pp;
+            cache[kHeavyCache + i] = x.base;
+            cache[2 * kHeavyCache + i] = x.k;
+        }
+        __syncthreads();
         auto count = [&](uint32_t t) {
-            uint64_t const c = count_at_least(a, e, q0, q1, t, nullptr, s_incl, s_base, s_k, uint64_t(wv) * kWave, kHT);
+            uint64_t const c = count_at_least(a, e, q0, q1, t, cache, s_incl, s_base, s_k, uint64_t(wv) * kWave, kHT,
+                                              kHeavyCache);
             if (lane == 0) s_part[wv] = c;
             __syncthreads();
             uint64_t tot = 0;
@@ -589,7 +601,7 @@ __global__ __launch_bounds__(kHT) void k_hidx_round(HidxArgs a) {
     uint32_t* hist = s_hist[wv];
     bool changed = false;
     if (blockIdx.x < a.heavy_blocks) {
-        hidx_heavy(a, s_incl[wv], s_base[wv], s_skip[wv], reinterpret_cast<uint64_t*>(s_pp));
+        hidx_heavy(a, &s_hist[0][0], s_incl[wv], s_base[wv], s_skip[wv], reinterpret_cast<uint64_t*>(s_pp));
         return;
     }
     uint32_t const nw = (gridDim.x - a.heavy_blocks) * kHW;
