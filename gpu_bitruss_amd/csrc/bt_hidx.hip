@@ -75,6 +75,7 @@ constexpr uint32_t kCacheInc = 256;   // incidences whose bloom data a wave keep
 constexpr uint32_t kSearchCost = 12;  // cost of a binary search in one chunk, in values streamed
 constexpr uint64_t kHeavyInc = 256;   // edges with more incidences: a workgroup each (sorted chunks only)
 constexpr uint32_t kHeavyCache = kHW * kHB / 3;  // incidences a heavy edge's workgroup caches in its histograms
+constexpr uint32_t kHeavyGlobal = 61440;         // and the next ones in its own HBM rows (hcache)
 
 // Device-side loop state (control words of the graph): round counter,
 // change flag of the current round, done flag.
@@ -297,6 +298,7 @@ struct HidxArgs {
     const uint32_t* heavy;
     uint32_t n_heavy, heavy_blocks;
     unsigned long long* heavy_next;
+    uint32_t* hcache;  // per heavy workgroup: 3 rows of kHeavyGlobal (pp, base, k)
     // graph rounds: the device loop state (null: round and dirty_in above
     // hold); the round number is read from it, and round 1 ignores dirty_in
     const unsigned long long* dev_round;
@@ -400,7 +402,8 @@ __device__ __forceinline__ uint32_t item_count(const uint32_t* __restrict__ smv,
 // the wave. Each bloom with prev[partner] >= t gives its mv values >= t less
 // the edge's own middle (mv = min(prev[e], partner) >= t). The wave takes
 // 64 incidences at a time (the bloom data of the first kCacheInc from its
-// LDS cache, if any: pp, base and k in three rows of cap) and
+// LDS cache, if any: pp, base and k in three rows of cap, then those of
+// the next kHeavyGlobal from gcache, if any) and
 // spreads their items (a small bloom, or one chunk of a larger one) over
 // its lanes, so that the chunks of one large bloom are searched side by
 // side. s_incl, s_base and s_k are the wave's 64-entry LDS rows. A wave of
@@ -408,7 +411,8 @@ __device__ __forceinline__ uint32_t item_count(const uint32_t* __restrict__ smv,
 // on, every stride, and returns its own share.
 __device__ uint64_t count_at_least(const HidxArgs& a, uint32_t e, uint64_t q0, uint64_t q1, uint32_t t,
                                    const uint32_t* cache, uint32_t* s_incl, uint32_t* s_base, uint32_t* s_k,
-                                   uint64_t first = 0, uint64_t stride = kWave, uint32_t cap = kCacheInc) {
+                                   uint64_t first = 0, uint64_t stride = kWave, uint32_t cap = kCacheInc,
+                                   const uint32_t* gcache = nullptr) {
     int const lane = threadIdx.x & (kWave - 1);
     uint64_t c = 0, own = 0;
     for (uint64_t b0 = q0 + first; b0 < q1; b0 += stride) {
@@ -416,7 +420,12 @@ __device__ uint64_t count_at_least(const HidxArgs& a, uint32_t e, uint64_t q0, u
         IncInfo x{0u, 0u, 0u};
         if (q < q1) {
             uint64_t const i = q - q0;
-            x = cache && i < cap ? IncInfo{cache[i], cache[cap + i], cache[2 * cap + i]} : inc_info(a, e, q);
+            if (cache && i < cap)
+                x = IncInfo{cache[i], cache[cap + i], cache[2 * cap + i]};
+            else if (gcache && i >= cap && i - cap < kHeavyGlobal)
+                x = IncInfo{gcache[i - cap], gcache[kHeavyGlobal + i - cap], gcache[2 * kHeavyGlobal + i - cap]};
+            else
+                x = inc_info(a, e, q);
         }
         uint32_t const items = x.pp >= t ? (x.k <= kSmallBloom ? 1u : (x.k + kChunk - 1) / kChunk) : 0u;
         own += __builtin_popcountll(__ballot(items != 0));
@@ -541,18 +550,25 @@ __device__ void hidx_heavy(const HidxArgs& a, uint32_t* cache, uint32_t* s_incl,
             if (threadIdx.x == 0) a.emark[e] = 0;
         }
         uint64_t const q0 = a.inc_off[e], q1 = a.inc_off[e + 1];
-        uint64_t const qc = std::min(q1, q0 + kHeavyCache);
+        uint64_t const qc = std::min(q1, q0 + kHeavyCache + kHeavyGlobal);
+        uint32_t* gcache = a.hcache + size_t(blockIdx.x) * 3 * kHeavyGlobal;
         for (uint64_t q = q0 + threadIdx.x; q < qc; q += kHT) {
             IncInfo const x = inc_info(a, e, q);
             uint64_t const i = q - q0;
-            cache[i] = x.pp;
-            cache[kHeavyCache + i] = x.base;
-            cache[2 * kHeavyCache + i] = x.k;
+            if (i < kHeavyCache) {
+                cache[i] = x.pp;
+                cache[kHeavyCache + i] = x.base;
+                cache[2 * kHeavyCache + i] = x.k;
+            } else {
+                gcache[i - kHeavyCache] = x.pp;
+                gcache[kHeavyGlobal + i - kHeavyCache] = x.base;
+                gcache[2 * kHeavyGlobal + i - kHeavyCache] = x.k;
+            }
         }
         __syncthreads();
         auto count = [&](uint32_t t) {
             uint64_t const c = count_at_least(a, e, q0, q1, t, cache, s_incl, s_base, s_k, uint64_t(wv) * kWave, kHT,
-                                              kHeavyCache);
+                                              kHeavyCache, gcache);
             if (lane == 0) s_part[wv] = c;
             __syncthreads();
             uint64_t tot = 0;
@@ -703,6 +719,7 @@ struct SortedChunks {
     unsigned grid_small = 1;          // the small blooms' sort
     const uint32_t* heavy = nullptr;  // the heavy edges and their workgroups
     uint32_t n_heavy = 0, heavy_blocks = 0;
+    uint32_t* hcache = nullptr;  // the workgroups' HBM rows of bloom data
 };
 
 // smv lives in the peel's claim stamps (bclaim, at least R u32) and the
@@ -755,6 +772,10 @@ static int hidx_sorted_setup(bt_dgraph* dg, SortedChunks* sc) {
         sc->heavy = static_cast<const uint32_t*>(ix.live.ptr);
         sc->n_heavy = nh;
         sc->heavy_blocks = std::min<uint32_t>(nh, uint32_t(c->cu_count));
+        // the rows in the peel's sorted both-frontier lists (c2s, unused by
+        // h-index rounds)
+        BT_TRY(bt_buf_reserve(ix.c2s, sizeof(uint32_t) * 3 * size_t(kHeavyGlobal) * sc->heavy_blocks));
+        sc->hcache = static_cast<uint32_t*>(ix.c2s.ptr);
     }
     return BT_OK;
 }
@@ -822,6 +843,7 @@ static int hidx_graph_loop(bt_dgraph* dg, uint64_t* mv, const SortedChunks& sc, 
                    sc.n_heavy,
                    sc.heavy_blocks,
                    st.work,
+                   sc.hcache,
                    st.round,
                    st.done};
         k_hidx_round<<<grid_e + sc.heavy_blocks, kHT, 0, s>>>(a);
@@ -949,6 +971,7 @@ int bt_hidx_index(bt_dgraph* dg, uint32_t* rounds_out) {
                        sc.n_heavy,
                        sc.heavy_blocks,
                        work,
+                       sc.hcache,
                        nullptr,
                        nullptr};
             k_hidx_round<<<grid_e + sc.heavy_blocks, kHT, 0, s>>>(a);
