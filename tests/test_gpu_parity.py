@@ -453,7 +453,7 @@ def test_skewed_60m_msp_whole(dev, tmp_path):
 @pytest.mark.timeout(1000)
 def test_skewed_60m_hidx_whole(dev, tmp_path):
     """The 60M-edge graph's h-index decomposition to its fixed point
-    (`coh --gpu --algo hidx`, bitruss_hidx.cu:179-256; about 300 s with the
+    (`coh --gpu --algo hidx`, bitruss_hidx.cu:179-256; about 200 s with the
     sorted chunks) against the bitruss numbers of the oracle's exact bloom
     peel (or_msp_bloom with BLOOM_EXACT, which equals the textbook peel
     cpu_bitruss.cpp:34-122 on every graph the CPU tests hold): the Jacobi
