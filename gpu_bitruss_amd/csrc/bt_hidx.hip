@@ -32,14 +32,13 @@
 // previous round. The number of an edge's values >= t is then, per bloom
 // with prev[partner] >= t, a binary search per chunk (the chunks of one
 // bloom spread over the wave) minus the edge's own middle, and the h-index
-// is found by a
-// search over t after a first count at t = prev[e] (most edges keep their
-// value in the late rounds), each count narrowing the range from both
-// sides. An edge none of whose blooms changed in the previous round keeps
-// its value; the marks come per edge from the changed blooms' records.
-// Per edge the wave
-// estimates both costs from its blooms' sizes and takes the cheaper way;
-// both give the same h-index.
+// is found by a search over t after a first count at t = prev[e] (most
+// edges keep their value in the late rounds), each count narrowing the
+// range from both sides. An edge is evaluated again only when a value in
+// one of its blooms dropped from at least its h-index (k_hidx_mark). Per
+// edge the wave estimates both costs from its blooms' sizes and takes the
+// cheaper way; both give the same h-index. Edges in more than kHeavyInc
+// blooms are evaluated by a whole workgroup each (hidx_heavy).
 //
 // Round loop: the reference reads a change flag back after every round
 // (bitruss_hidx.cu:219-231). Here the rounds are captured once into a HIP
