@@ -770,7 +770,7 @@ static int hidx_sorted_setup(bt_dgraph* dg, SortedChunks* sc) {
         BT_HIP_TRY(hipGetLastError());
         sc->heavy = static_cast<const uint32_t*>(ix.live.ptr);
         sc->n_heavy = nh;
-        sc->heavy_blocks = std::min<uint32_t>(nh, uint32_t(c->cu_count));
+        sc->heavy_blocks = std::min<uint32_t>(nh, uint32_t(c->cu_count) * 2);
         // the rows in the peel's sorted both-frontier lists (c2s, unused by
         // h-index rounds)
         BT_TRY(bt_buf_reserve(ix.c2s, sizeof(uint32_t) * 3 * size_t(kHeavyGlobal) * sc->heavy_blocks));
